@@ -1,0 +1,181 @@
+/*
+ * hexa.h -- C ABI of libhexa, the MI355X (gfx950) implementation of HEXA
+ * TopOpt's inner solve loop: matrix-free hex8 K(rho)*u, the PCG driver with
+ * a geometric-multigrid V-cycle, the Helmholtz density filter, the stress
+ * "sensitivity" with the reference density update, and SIMP + OC.
+ *
+ * Reference interfaces replaced (R/ = v119_2025_12_19_HEXATOP_AWS/):
+ *   GPUSolver.solve_system_gpu            R/src/Solvers/GPUSolver.jl:467-477
+ *     (and gpu_matrix_free_cg_solve       R/src/Solvers/GPUSolver.jl:182-465)
+ *   GPUGeometricMultigrid.setup_multigrid / apply_mg_vcycle!
+ *                                         R/src/Solvers/GPUGeometricMultigrid.jl:253-346
+ *   GPUHelmholtz.apply_gpu_filter!        R/src/Optimization/GPUHelmholtz.jl:422-443
+ *   Stress.compute_stress_field           R/src/Core/Stress.jl:287-374
+ *   TopologyOptimization.update_density!  R/src/Optimization/TopOpt.jl:211-482
+ *   compliance dot(F, U)                  R/src/Main.jl:453
+ * plus SIMP sensitivity / OC (no reference counterpart), batched load cases
+ * and a device-resident job server.
+ *
+ * Conventions
+ *   - Structured grid of nx*ny*nz bricks of size dx*dy*dz (the reference's
+ *     generate_mesh, R/src/Mesh/Mesh.jl:23-60), 0-based:
+ *       node (i,j,k)    = i + (nx+1)*(j + (ny+1)*k)
+ *       element (i,j,k) = i + nx*(j + ny*k)
+ *       DOF             = 3*node + component   (interleaved, as R's U and F)
+ *   - A pruned reference mesh (R/src/Mesh/MeshPruner.jl:10-87) is passed as
+ *     the full box plus an active-element mask; nodes that no active element
+ *     touches are dropped from the system, which is the same linear system
+ *     as the renumbered one.
+ *   - Every function returns an int32 status (HEXA_OK or a negative code);
+ *     hexa_last_error() returns a thread-local message.  No function calls
+ *     exit/abort.  Host buffers are borrowed for the duration of a call.
+ *   - A handle owns its device state and one HIP stream; use one handle from
+ *     one thread at a time.  Distinct handles are independent.
+ *   - Functions with the _dev suffix take device pointers that must stay
+ *     valid until the call returns; the call is ordered on the handle's
+ *     stream and synchronises before returning unless stated otherwise.
+ */
+#ifndef HEXA_H
+#define HEXA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define HEXA_OK               0
+#define HEXA_E_INVALID       (-1)  /* bad argument / shape / state            */
+#define HEXA_E_OOM           (-2)  /* device memory (or budget) exhausted     */
+#define HEXA_E_NAN_INPUT     (-3)  /* NaN/Inf in the stiffness diagonal       */
+#define HEXA_E_NOT_CONVERGED (-4)  /* solve stopped above tolerance           */
+#define HEXA_E_DEVICE        (-5)  /* HIP runtime error / no device           */
+
+/* ---- enumerations ---------------------------------------------------- */
+enum { HEXA_PREC_FP64 = 0, HEXA_PREC_MIXED = 1 };            /* V-cycle precision */
+enum { HEXA_KE_FP64 = 0, HEXA_KE_F32_WIDENED = 1 };           /* element matrix    */
+enum { HEXA_HELM_BUG_COMPAT = 0, HEXA_HELM_CORRECTED = 1 };    /* filter element    */
+enum { HEXA_PRECOND_JACOBI = 0, HEXA_PRECOND_MG = 1 };
+enum { HEXA_MATVEC_AUTO = 0, HEXA_MATVEC_DENSE = 1, HEXA_MATVEC_FACTORED = 2 };
+enum { HEXA_SEMANTICS_GPU = 0, HEXA_SEMANTICS_CPU = 1 };
+enum { HEXA_DIAG_EXACT = 0, HEXA_DIAG_X_ENTRY = 1 };
+/* method reported by a solve; names follow GPUSolver.jl:314,463 */
+enum { HEXA_METHOD_JACOBI_GPU = 0, HEXA_METHOD_GMG_GPU = 1,
+       HEXA_METHOD_FAILED_CORRUPT_INPUT = 2, HEXA_METHOD_JACOBI_CPU = 3 };
+
+typedef struct hexa_handle hexa_handle;
+
+typedef struct {
+    int32_t nx, ny, nz;        /* elements per axis (>= 1)                */
+    double dx, dy, dz;         /* element edge lengths (> 0)               */
+} hexa_grid;
+
+typedef struct {
+    /* element stiffness multiplier  Emin + rho^penal * (E0 - Emin);
+     * the reference's linear E*rho is E0=E, Emin=0, penal=1              */
+    double E0, Emin, penal, nu;
+    int32_t ke_mode;           /* HEXA_KE_*                                */
+    int32_t helm_element;      /* HEXA_HELM_*                              */
+    int32_t precision;         /* HEXA_PREC_*  (MG V-cycle arithmetic)      */
+    int32_t matvec_kernel;     /* HEXA_MATVEC_*                            */
+    int32_t mg_levels;         /* 0 = automatic                            */
+    int32_t mg_pre, mg_post;   /* damped-Jacobi sweeps                      */
+    double mg_omega;           /* Jacobi damping                            */
+    int32_t nrhs;              /* load cases solved together (>= 1)        */
+    int32_t diag_mode;         /* HEXA_DIAG_*                               */
+    int64_t reserved[8];
+} hexa_options;
+
+typedef struct {
+    double tol;                /* relative residual ||r||/||f_free||        */
+    int32_t max_iter;
+    double shift_factor;       /* s = shift_factor * max diag(K_ff)         */
+    int32_t max_shift_attempts;
+    double shift_multiplier;
+    double stagnation_tol;     /* accept best x below this (0 = off)       */
+    int32_t precond;           /* HEXA_PRECOND_*                           */
+    int32_t semantics;         /* HEXA_SEMANTICS_*                          */
+    int32_t warm_start;        /* use u_inout as the attempt-1 guess       */
+    int32_t check_every;       /* 0 = semantics default (GPU 10, CPU 1)    */
+    int64_t reserved[4];
+} hexa_solve_params;
+
+typedef struct {
+    double rel_res;            /* last checked ||r||/||f_free||             */
+    double true_rel_res;       /* ||f - A u||/||f|| recomputed in FP64      */
+    double shift_used;
+    double compliance;         /* f . u                                     */
+    double setup_ms, solve_ms;
+    int32_t iterations;
+    int32_t attempt;
+    int32_t method;            /* HEXA_METHOD_*                             */
+    int32_t converged;
+    int32_t mg_levels;
+    int32_t pad;
+    int64_t matvecs;
+} hexa_solve_report;
+
+/* ---- library / handle ------------------------------------------------- */
+const char* hexa_last_error(void);
+int32_t hexa_version(void);
+void hexa_default_options(hexa_options* opt);
+void hexa_default_solve_params(hexa_solve_params* prm);
+/* Device name (NUL-terminated, truncated to len), total bytes, CU count. */
+int32_t hexa_device_info(char* name, int32_t len, int64_t* total_bytes, int32_t* n_cu);
+
+/* Replaces the per-call workspace set-up of GPUSolver.jl:246-256 and the
+ * three module-global caches (GPUSolver.jl:63, GPUGeometricMultigrid.jl:37,
+ * GPUHelmholtz.jl:41) by one handle with device-resident state. */
+int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle** out);
+int32_t hexa_destroy(hexa_handle* h);
+
+/* Supports and loads.  fixed_dof: 3*nN bytes, nonzero = held (the
+ * reference's bc_indicator > 0, GPUSolver.jl:170-180).  f: nrhs * 3*nN,
+ * load case r at f + r*3*nN.  active_elem: nE bytes or NULL (all active). */
+int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double* f,
+                         const uint8_t* active_elem);
+
+/* Element densities (nE), FP64.  The stiffness multiplier is formed on the
+ * device (GPUSolver.jl:254 for the reference's linear law). */
+int32_t hexa_set_density(hexa_handle* h, const double* rho);
+int32_t hexa_set_density_dev(hexa_handle* h, const double* d_rho);
+
+/* Linear solve (K_ff(rho) + s I) u = f_free.  Replaces
+ * GPUSolver.solve_system_gpu (GPUSolver.jl:467-477).  u_inout: nrhs*3*nN
+ * host buffer, read as the warm start when prm->warm_start, written with
+ * the solution (held DOFs = 0).  May be NULL: the solution then stays on
+ * the device (hexa_get_displacement).  Returns HEXA_OK when every load
+ * case reached tol, HEXA_E_NOT_CONVERGED when the best iterate was
+ * returned instead (GPUSolver.jl:439-443), HEXA_E_NAN_INPUT for a NaN/Inf
+ * diagonal (GPUSolver.jl:310-315; u is then zero).  rep may be NULL; for
+ * nrhs > 1 rep points to nrhs reports. */
+int32_t hexa_solve(hexa_handle* h, double* u_inout, const hexa_solve_params* prm,
+                   hexa_solve_report* rep);
+int32_t hexa_get_displacement(hexa_handle* h, double* u);
+/* Device pointer of the solution (nrhs*3*nN FP64), valid until the next
+ * solve or destroy. */
+int32_t hexa_displacement_dev(hexa_handle* h, const double** d_u);
+
+/* y = K(rho) x on all DOFs (no supports, no shift): the bare operator of
+ * matvec_unstructured_atomic_kernel! (GPUSolver.jl:91-128) on the full
+ * vector.  Host arrays of 3*nN. */
+int32_t hexa_matvec(hexa_handle* h, const double* x, double* y);
+/* Same on device arrays; precision HEXA_PREC_FP64 (double*) or
+ * HEXA_PREC_MIXED (float*); 'reps' back-to-back launches (for timing);
+ * no synchronisation. */
+int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t precision,
+                        int32_t reps);
+/* Apply the preconditioner of the last solve set-up to r (3*nN host
+ * arrays, FP64): z = M^-1 r. */
+int32_t hexa_apply_precond(hexa_handle* h, const double* r, double* z);
+
+/* Seconds of GPU time of the last hexa_matvec_dev batch (HIP events on the
+ * handle's stream), averaged per launch. */
+int32_t hexa_last_kernel_ms(hexa_handle* h, double* ms);
+int32_t hexa_stream_sync(hexa_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEXA_H */

@@ -1,0 +1,831 @@
+// libhexa C ABI: handle management and the linear-solve driver.
+//
+// The solve follows the control flow of the reference GPU driver
+// gpu_matrix_free_cg_solve (R/src/Solvers/GPUSolver.jl:182-465) or, with
+// HEXA_SEMANTICS_CPU, of the CPU matrix-free solver matrix_free_cg_solve
+// (R/src/Solvers/CPUSolver.jl:192-255), with every vector and scalar kept on
+// the device.  The multigrid preconditioner is a geometric V-cycle over the
+// structured hierarchy (damped Jacobi smoothing, trilinear transfers,
+// rediscretised coarse operators, dense direct solve on the coarsest grid),
+// in FP64 or in FP32 with BF16-stored smoother diagonals.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "hexa_internal.h"
+#include "ke.h"
+
+namespace hexa {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+namespace {
+
+constexpr int RING = 4;       // pinned state slots for the look-ahead poll
+constexpr int LOOKAHEAD = 2;  // iterations queued beyond the last one inspected
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct Level {
+    Grid3 g{};
+    double kscale = 1.0;            // Ke(level) = kscale * Ke(fine)
+    double shift = 0.0;
+    double* Ke64 = nullptr;
+    float* Ke32 = nullptr;
+    double* fac64 = nullptr;        // owned for l >= 1
+    float* fac32 = nullptr;
+    uint8_t* mask = nullptr;        // owned for l >= 1
+    double* diag = nullptr;
+    void* dinv = nullptr;
+    void* b = nullptr;              // rhs (owned for l >= 1, or l == 0 in mixed mode)
+    void* x = nullptr;
+    void* xt = nullptr;
+    void* r = nullptr;
+};
+
+}  // namespace
+}  // namespace hexa
+
+using namespace hexa;
+
+struct hexa_handle {
+    hexa_grid grid{};
+    hexa_options opt{};
+    Grid3 g{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipEvent_t ring_ev[RING] = {};
+    double last_kernel_ms = 0.0;
+
+    double Ke[576];
+    double* d_Ke64 = nullptr;
+    float* d_Ke32 = nullptr;
+
+    uint8_t* d_fixed = nullptr;    // 3*nn
+    uint8_t* d_active = nullptr;   // ne or null
+    uint8_t* d_mask = nullptr;     // nn
+    double* d_f = nullptr;         // nrhs * 3nn
+    double* d_rho = nullptr;
+    double* d_fac = nullptr;
+    float* d_fac32 = nullptr;
+    bool have_problem = false, have_density = false, fac32_valid = false;
+
+    double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_z = nullptr;
+    double *d_minv = nullptr, *d_best = nullptr, *d_b = nullptr, *d_diag = nullptr, *d_guess = nullptr;
+    double* d_partials = nullptr;
+    double* d_scal = nullptr;
+    PcgState* d_state = nullptr;
+    PcgState* h_state = nullptr;   // pinned ring
+
+    // preconditioner set-up of the last solve
+    int precond_ready = -1;        // HEXA_PRECOND_* or -1
+    std::vector<Level> levels;
+    double* d_Ainv = nullptr;
+    double *d_gj_row = nullptr, *d_gj_col = nullptr;
+    int64_t n_coarse = 0;
+    double shift = 0.0;
+
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int32_t dalloc(hexa_handle* h, void** p, size_t bytes) {
+    if (*p) return HEXA_OK;
+    hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        return HEXA_E_OOM;
+    }
+    h->allocs.push_back(*p);
+    return HEXA_OK;
+}
+template <typename T>
+int32_t dalloc(hexa_handle* h, T** p, size_t count) {
+    return dalloc(h, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+int32_t check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return HEXA_E_DEVICE;
+    }
+    return HEXA_OK;
+}
+
+int32_t ensure_fac32(hexa_handle* h) {
+    if (h->fac32_valid) return HEXA_OK;
+    HX_TRY(dalloc(h, &h->d_fac32, (size_t)h->g.ne));
+    HX_TRY(dalloc(h, &h->d_Ke32, 576));
+    float k32[576];
+    for (int i = 0; i < 576; ++i) k32[i] = (float)h->Ke[i];
+    HX_CHECK(hipMemcpyAsync(h->d_Ke32, k32, sizeof k32, hipMemcpyHostToDevice, h->stream));
+    launch_convert<double, float>(h->stream, h->g.ne, h->d_fac, h->d_fac32);
+    HX_TRY(check_launch());
+    HX_CHECK(hipStreamSynchronize(h->stream));
+    h->fac32_valid = true;
+    return HEXA_OK;
+}
+
+// ------------------------------------------------------------------ MG
+int choose_levels(const hexa_handle* h) {
+    const int want = h->opt.mg_levels;
+    int nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
+    int L = 1;
+    if (want > 0) {
+        while (L < want && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && nx >= 2 && ny >= 2 && nz >= 2) {
+            nx /= 2; ny /= 2; nz /= 2; ++L;
+        }
+        return L;
+    }
+    while (nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 &&
+           3LL * (nx + 1) * (ny + 1) * (nz + 1) > 2000) {
+        nx /= 2; ny /= 2; nz /= 2; ++L;
+    }
+    return L;
+}
+
+bool mixed(const hexa_handle* h) { return h->opt.precision == HEXA_PREC_MIXED; }
+
+// Builds the hierarchy's shift-independent part: grids, Ke, factors, masks,
+// diagonals, work vectors.
+int32_t mg_build(hexa_handle* h) {
+    const int L = choose_levels(h);
+    const bool mx = mixed(h);
+    h->levels.assign(L, Level{});
+    for (int l = 0; l < L; ++l) {
+        Level& lv = h->levels[l];
+        lv.g = make_grid(h->g.nx >> l, h->g.ny >> l, h->g.nz >> l);
+        lv.kscale = std::ldexp(1.0, l);
+        const int64_t nd = 3 * lv.g.nn;
+        double ke[576];
+        for (int i = 0; i < 576; ++i) ke[i] = lv.kscale * h->Ke[i];
+        HX_TRY(dalloc(h, &lv.Ke64, 576));
+        HX_CHECK(hipMemcpyAsync(lv.Ke64, ke, sizeof ke, hipMemcpyHostToDevice, h->stream));
+        if (mx) {
+            float k32[576];
+            for (int i = 0; i < 576; ++i) k32[i] = (float)ke[i];
+            HX_TRY(dalloc(h, &lv.Ke32, 576));
+            HX_CHECK(hipMemcpyAsync(lv.Ke32, k32, sizeof k32, hipMemcpyHostToDevice, h->stream));
+        }
+        if (l == 0) {
+            lv.fac64 = h->d_fac;
+            lv.mask = h->d_mask;
+        } else {
+            Level& up = h->levels[l - 1];
+            HX_TRY(dalloc(h, &lv.fac64, (size_t)lv.g.ne));
+            HX_TRY(dalloc(h, &lv.mask, (size_t)lv.g.nn));
+            launch_coarsen_factor<double>(h->stream, up.g, up.fac64, lv.fac64);
+            launch_coarsen_mask(h->stream, up.g, lv.g, up.mask, lv.mask);
+        }
+        if (mx) {
+            if (l == 0) {
+                HX_TRY(ensure_fac32(h));
+                lv.fac32 = h->d_fac32;
+            } else {
+                HX_TRY(dalloc(h, &lv.fac32, (size_t)lv.g.ne));
+                launch_convert<double, float>(h->stream, lv.g.ne, lv.fac64, lv.fac32);
+            }
+        }
+        HX_TRY(dalloc(h, &lv.diag, (size_t)nd));
+        launch_diag<double>(h->stream, lv.g, lv.Ke64, lv.fac64, lv.mask, 0, lv.diag);
+        const size_t es = mx ? sizeof(float) : sizeof(double);
+        const bool coarsest = (l == L - 1);
+        if (!coarsest) {
+            HX_TRY(dalloc(h, &lv.dinv, (size_t)nd * (mx ? sizeof(uint16_t) : sizeof(double))));
+            HX_TRY(dalloc(h, &lv.xt, nd * es));
+            HX_TRY(dalloc(h, &lv.r, nd * es));
+        }
+        if (l > 0 || mx) HX_TRY(dalloc(h, &lv.b, nd * es));
+        if (l > 0 || mx) HX_TRY(dalloc(h, &lv.x, nd * es));
+        HX_TRY(check_launch());
+    }
+    Level& c = h->levels[L - 1];
+    h->n_coarse = 3 * c.g.nn;
+    if (h->n_coarse > 20000) {
+        set_error("coarsest multigrid level has %lld DOFs (> 20000); use more levels",
+                  (long long)h->n_coarse);
+        return HEXA_E_INVALID;
+    }
+    HX_TRY(dalloc(h, &h->d_Ainv, (size_t)(h->n_coarse * h->n_coarse)));
+    HX_TRY(dalloc(h, &h->d_gj_row, (size_t)h->n_coarse));
+    HX_TRY(dalloc(h, &h->d_gj_col, (size_t)h->n_coarse));
+    return HEXA_OK;
+}
+
+// Shift-dependent part: smoother diagonals and the coarsest inverse.  The
+// shift s of the fine operator is carried to level l as 8^l s (row sum of
+// P^T P for the trilinear transfer).
+int32_t mg_set_shift(hexa_handle* h, double s) {
+    const bool mx = mixed(h);
+    const int L = (int)h->levels.size();
+    for (int l = 0; l < L; ++l) {
+        Level& lv = h->levels[l];
+        lv.shift = s * std::pow(8.0, l);
+        if (l < L - 1) {
+            if (mx) launch_make_dinv<uint16_t>(h->stream, 3 * lv.g.nn, lv.diag, lv.mask, lv.shift, 0,
+                                               (uint16_t*)lv.dinv);
+            else launch_make_dinv<double>(h->stream, 3 * lv.g.nn, lv.diag, lv.mask, lv.shift, 0,
+                                          (double*)lv.dinv);
+        }
+    }
+    Level& c = h->levels[L - 1];
+    launch_fill<double>(h->stream, h->n_coarse * h->n_coarse, 0.0, h->d_Ainv);
+    launch_coarse_assemble(h->stream, c.g, c.Ke64, c.fac64, c.mask, c.shift, h->d_Ainv);
+    launch_gauss_jordan(h->stream, h->n_coarse, h->d_Ainv, h->d_gj_row, h->d_gj_col);
+    HX_TRY(check_launch());
+    return HEXA_OK;
+}
+
+template <typename T, typename DT>
+void vcycle_level(hexa_handle* h, int l, const T* b, T* out) {
+    Level& lv = h->levels[l];
+    const int L = (int)h->levels.size();
+    hipStream_t s = h->stream;
+    const int64_t nd = 3 * lv.g.nn;
+    if (l == L - 1) {
+        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out);
+        return;
+    }
+    const T* Ke = std::is_same<T, float>::value ? (const T*)lv.Ke32 : (const T*)lv.Ke64;
+    const T* fac = std::is_same<T, float>::value ? (const T*)lv.fac32 : (const T*)lv.fac64;
+    const DT* dinv = (const DT*)lv.dinv;
+    const T omega = (T)h->opt.mg_omega;
+    const T shift = (T)lv.shift;
+    const int pre = std::max(1, h->opt.mg_pre), post = std::max(0, h->opt.mg_post);
+    const int swaps = (pre - 1) + post;
+    T* xa = (swaps % 2 == 0) ? out : (T*)lv.xt;
+    T* xb = (swaps % 2 == 0) ? (T*)lv.xt : out;
+    launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa);
+    for (int it = 1; it < pre; ++it) {
+        launch_elastic_op<T, DT>(s, lv.g, OP_SMOOTH, Ke, fac, lv.mask, xa, b, dinv, omega, shift, xb);
+        std::swap(xa, xb);
+    }
+    launch_elastic_op<T, DT>(s, lv.g, OP_RESID, Ke, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r);
+    Level& nx = h->levels[l + 1];
+    launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b);
+    vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x);
+    launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa);
+    for (int it = 0; it < post; ++it) {
+        launch_elastic_op<T, DT>(s, lv.g, OP_SMOOTH, Ke, fac, lv.mask, xa, b, dinv, omega, shift, xb);
+        std::swap(xa, xb);
+    }
+}
+
+// z = M^-1 r (FP64 in and out).
+void apply_mg(hexa_handle* h, const double* r, double* z) {
+    if (h->levels.size() == 1) {
+        launch_dense_gemv<double>(h->stream, h->n_coarse, h->d_Ainv, r, z);
+        return;
+    }
+    if (mixed(h)) {
+        Level& l0 = h->levels[0];
+        const int64_t nd = 3 * l0.g.nn;
+        launch_convert<double, float>(h->stream, nd, r, (float*)l0.b);
+        vcycle_level<float, uint16_t>(h, 0, (const float*)l0.b, (float*)l0.x);
+        launch_convert<float, double>(h->stream, nd, (const float*)l0.x, z);
+    } else {
+        vcycle_level<double, double>(h, 0, r, z);
+    }
+}
+
+int32_t ensure_solver_vectors(hexa_handle* h) {
+    const size_t nd = (size_t)(3 * h->g.nn);
+    HX_TRY(dalloc(h, &h->d_x, nd));
+    HX_TRY(dalloc(h, &h->d_r, nd));
+    HX_TRY(dalloc(h, &h->d_p, nd));
+    HX_TRY(dalloc(h, &h->d_Ap, nd));
+    HX_TRY(dalloc(h, &h->d_z, nd));
+    HX_TRY(dalloc(h, &h->d_minv, nd));
+    HX_TRY(dalloc(h, &h->d_best, nd));
+    HX_TRY(dalloc(h, &h->d_b, nd));
+    HX_TRY(dalloc(h, &h->d_diag, nd));
+    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
+    HX_TRY(dalloc(h, &h->d_scal, 16));
+    HX_TRY(dalloc(h, &h->d_state, 1));
+    return HEXA_OK;
+}
+
+// b = f masked to the free DOFs.
+__global__ void k_mask_copy(int64_t nd, const double* f, const uint8_t* __restrict__ mask, double* b) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = ((mask[i / 3] >> (i % 3)) & 1) ? 0.0 : f[i];
+}
+
+unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 65535 * 8) b = 65535 * 8;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+struct AttemptResult {
+    int halted = PCG_RUNNING;
+    int iterations = 0;
+    double final_rel = 0.0, best_rel = INFINITY;
+};
+
+// One PCG attempt with shift s from the current d_x.  Returns the final
+// device state (synchronised).
+int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, double s,
+                    AttemptResult* res, int64_t* matvecs) {
+    hipStream_t st = h->stream;
+    const int64_t nd = 3 * h->g.nn;
+    const bool cpu = prm->semantics == HEXA_SEMANTICS_CPU;
+    const int check_every = prm->check_every > 0 ? prm->check_every : (cpu ? 1 : 10);
+
+    // minv and the MG shift
+    launch_make_dinv<double>(st, nd, h->d_diag, h->d_mask, s, cpu ? 1 : 0, h->d_minv);
+    if (use_mg) HX_TRY(mg_set_shift(h, s));
+
+    // state, r = b - A x, z = M^-1 r, p = z, rz
+    launch_dot2(st, nd, h->d_b, h->d_b, h->d_partials);
+    launch_pcg_init_state(st, h->d_state, h->d_partials, prm->tol);
+    launch_elastic_op<double, double>(st, h->g, OP_RESID, h->d_Ke64, h->d_fac, h->d_mask, h->d_x, h->d_b,
+                                      nullptr, 0.0, s, h->d_r);
+    ++*matvecs;
+    if (use_mg) apply_mg(h, h->d_r, h->d_z);
+    else launch_scale_dinv<double, double>(st, nd, h->d_r, h->d_minv, 1.0, h->d_z);
+    HX_CHECK(hipMemcpyAsync(h->d_p, h->d_z, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
+    launch_pcg_rz(st, nd, h->d_r, h->d_z, h->d_partials);
+    launch_pcg_set_rz(st, h->d_state, h->d_partials);
+    HX_TRY(check_launch());
+
+    int k_done = 0;
+    bool stop = false;
+    int inspected = 0;
+    for (int k = 1; k <= prm->max_iter && !stop; ++k) {
+        launch_elastic_op<double, double>(st, h->g, OP_APPLY, h->d_Ke64, h->d_fac, h->d_mask, h->d_p,
+                                          nullptr, nullptr, 0.0, s, h->d_Ap);
+        ++*matvecs;
+        launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
+        launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
+        launch_pcg_update(st, nd, h->d_state, h->d_x, h->d_p, h->d_r, h->d_Ap, use_mg ? nullptr : h->d_minv,
+                          h->d_partials);
+        const int check = cpu ? 1 : ((k % check_every == 0) || k == 1);
+        launch_pcg_check(st, h->d_state, h->d_partials, k, check, cpu ? 1 : 0, use_mg ? 0 : 1);
+        if (check && !cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
+        if (use_mg) {
+            apply_mg(h, h->d_r, h->d_z);
+            launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);
+            launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr);
+        } else {
+            launch_pcg_p_update(st, nd, h->d_state, h->d_p, nullptr, h->d_r, h->d_minv);
+        }
+        HX_TRY(check_launch());
+        const int slot = k % RING;
+        HX_CHECK(hipMemcpyAsync(&h->h_state[slot], h->d_state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+        HX_CHECK(hipEventRecord(h->ring_ev[slot], st));
+        k_done = k;
+        if (k - LOOKAHEAD >= 1) {
+            const int kk = k - LOOKAHEAD;
+            HX_CHECK(hipEventSynchronize(h->ring_ev[kk % RING]));
+            inspected = kk;
+            if (h->h_state[kk % RING].halted != PCG_RUNNING) stop = true;
+        }
+    }
+    (void)inspected;
+    HX_CHECK(hipMemcpyAsync(&h->h_state[0], h->d_state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    const PcgState& fs = h->h_state[0];
+    res->halted = fs.halted;
+    res->iterations = fs.halted != PCG_RUNNING ? fs.last_k : k_done;
+    res->final_rel = fs.final_rel;
+    res->best_rel = fs.best_rel;
+    return HEXA_OK;
+}
+
+int32_t need_ready(hexa_handle* h) {
+    if (!h) { set_error("null handle"); return HEXA_E_INVALID; }
+    if (!h->have_problem) { set_error("hexa_set_problem has not been called"); return HEXA_E_INVALID; }
+    if (!h->have_density) { set_error("hexa_set_density has not been called"); return HEXA_E_INVALID; }
+    return HEXA_OK;
+}
+
+int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_params* prm,
+                  hexa_solve_report* rep) {
+    hipStream_t st = h->stream;
+    const int64_t nd = 3 * h->g.nn;
+    const bool cpu = prm->semantics == HEXA_SEMANTICS_CPU;
+    hexa_solve_report R{};
+    R.method = cpu ? HEXA_METHOD_JACOBI_CPU : HEXA_METHOD_JACOBI_GPU;
+    const double t0 = now_ms();
+
+    // diagonal and its maximum over free DOFs (GPUSolver.jl:297-315)
+    const int xentry = h->opt.diag_mode == HEXA_DIAG_X_ENTRY && !cpu;
+    launch_diag<double>(st, h->g, h->d_Ke64, h->d_fac, h->d_mask, xentry, h->d_diag);
+    launch_max_free(st, h->g, h->d_diag, h->d_mask, h->d_partials, h->d_scal);
+    k_mask_copy<<<grid_for(nd), 256, 0, st>>>(nd, h->d_f + rhs * nd, h->d_mask, h->d_b);
+    HX_TRY(check_launch());
+    double maxd = 0.0;
+    HX_CHECK(hipMemcpyAsync(&maxd, h->d_scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    if (std::isnan(maxd) || std::isinf(maxd)) {
+        launch_fill<double>(st, nd, 0.0, h->d_x);
+        HX_CHECK(hipStreamSynchronize(st));
+        if (u_inout) std::memset(u_inout, 0, sizeof(double) * nd);
+        R.rel_res = NAN;
+        R.method = HEXA_METHOD_FAILED_CORRUPT_INPUT;
+        if (rep) *rep = R;
+        set_error("stiffness diagonal contains NaN/Inf");
+        return HEXA_E_NAN_INPUT;
+    }
+
+    const bool want_mg = prm->precond == HEXA_PRECOND_MG;
+    bool use_mg = want_mg;
+    if (use_mg && (h->precond_ready != HEXA_PRECOND_MG)) {
+        int32_t s = mg_build(h);
+        if (s != HEXA_OK) {
+            if (s == HEXA_E_OOM) return s;
+            use_mg = false;  // MG set-up failure falls back to Jacobi (GPUSolver.jl:259-267)
+        } else {
+            h->precond_ready = HEXA_PRECOND_MG;
+        }
+    }
+    if (prm->warm_start && u_inout) {
+        HX_TRY(dalloc(h, &h->d_guess, (size_t)nd));
+        HX_CHECK(hipMemcpyAsync(h->d_guess, u_inout, nd * sizeof(double), hipMemcpyHostToDevice, st));
+        k_mask_copy<<<grid_for(nd), 256, 0, st>>>(nd, h->d_guess, h->d_mask, h->d_guess);
+    }
+    launch_fill<double>(st, nd, 0.0, h->d_x);
+    launch_fill<double>(st, nd, 0.0, h->d_best);
+    HX_CHECK(hipStreamSynchronize(st));
+    R.setup_ms = now_ms() - t0;
+    const double t1 = now_ms();
+
+    int64_t matvecs = 0;
+    bool success = false;
+    double final_rel = 0.0;
+    int iters = 0, attempt_used = 1;
+    double shift_used = 0.0;
+    if (cpu) {
+        // CPUSolver.jl:192-255: one attempt, x0 = 0, clamp, every-iteration check
+        const double s = prm->shift_factor * maxd;
+        AttemptResult ar;
+        HX_TRY(run_attempt(h, prm, use_mg, s, &ar, &matvecs));
+        success = ar.halted == PCG_CONVERGED;
+        final_rel = ar.final_rel;
+        iters = ar.iterations;
+        shift_used = s;
+    } else {
+        // GPUSolver.jl:332-444
+        while (true) {
+            double cur = prm->shift_factor;
+            bool ok = false, fallback = false;
+            const int attempts = std::max(1, prm->max_shift_attempts);
+            double best_rel = INFINITY;
+            for (int attempt = 1; attempt <= attempts; ++attempt) {
+                const double s = cur * maxd;
+                attempt_used = attempt;
+                shift_used = s;
+                if (prm->warm_start && u_inout && attempt == 1 && use_mg == want_mg) {
+                    HX_CHECK(hipMemcpyAsync(h->d_x, h->d_guess, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
+                } else if (attempt > 1 || use_mg != want_mg) {
+                    launch_fill<double>(st, nd, 0.0, h->d_x);
+                }
+                AttemptResult ar;
+                HX_TRY(run_attempt(h, prm, use_mg, s, &ar, &matvecs));
+                iters = ar.iterations;
+                final_rel = ar.final_rel;
+                best_rel = ar.best_rel;
+                const bool diverged = ar.halted == PCG_NAN || ar.halted == PCG_DIVERGED;
+                if (diverged && use_mg) { fallback = true; break; }
+                if (ar.halted == PCG_CONVERGED) { ok = true; break; }
+                if (diverged && attempt != attempts) { cur *= prm->shift_multiplier; continue; }
+                if (prm->stagnation_tol > 0.0 && best_rel < prm->stagnation_tol) {
+                    ok = true;
+                    HX_CHECK(hipMemcpyAsync(h->d_x, h->d_best, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    break;
+                }
+                cur *= prm->shift_multiplier;
+            }
+            if (fallback) { use_mg = false; continue; }
+            success = ok;
+            if (!ok) HX_CHECK(hipMemcpyAsync(h->d_x, h->d_best, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
+            break;
+        }
+        R.method = use_mg ? HEXA_METHOD_GMG_GPU : HEXA_METHOD_JACOBI_GPU;
+    }
+    HX_CHECK(hipStreamSynchronize(st));
+    R.solve_ms = now_ms() - t1;
+
+    // compliance f.u and the true relative residual of the shifted system
+    launch_dot2(st, nd, h->d_b, h->d_x, h->d_partials);
+    launch_final_sum(st, 1, h->d_partials, h->d_scal);
+    launch_elastic_op<double, double>(st, h->g, OP_RESID, h->d_Ke64, h->d_fac, h->d_mask, h->d_x, h->d_b,
+                                      nullptr, 0.0, shift_used, h->d_r);
+    launch_dot2(st, nd, h->d_r, h->d_r, h->d_partials);
+    launch_final_sum(st, 1, h->d_partials, h->d_scal + 1);
+    launch_dot2(st, nd, h->d_b, h->d_b, h->d_partials);
+    launch_final_sum(st, 1, h->d_partials, h->d_scal + 2);
+    HX_TRY(check_launch());
+    double sc[3];
+    HX_CHECK(hipMemcpyAsync(sc, h->d_scal, sizeof sc, hipMemcpyDeviceToHost, st));
+    if (u_inout) HX_CHECK(hipMemcpyAsync(u_inout, h->d_x, nd * sizeof(double), hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    R.compliance = sc[0];
+    R.true_rel_res = sc[2] > 0 ? std::sqrt(sc[1] / sc[2]) : 0.0;
+    R.rel_res = final_rel;
+    R.iterations = iters;
+    R.attempt = attempt_used;
+    R.shift_used = shift_used;
+    R.converged = success ? 1 : 0;
+    R.matvecs = matvecs;
+    R.mg_levels = use_mg ? (int32_t)h->levels.size() : 0;
+    if (rep) *rep = R;
+    return success ? HEXA_OK : HEXA_E_NOT_CONVERGED;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* hexa_last_error(void) { return g_err.c_str(); }
+
+int32_t hexa_version(void) { return 100; }
+
+void hexa_default_options(hexa_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->E0 = 1.0;
+    o->Emin = 0.0;
+    o->penal = 1.0;
+    o->nu = 0.3;
+    o->ke_mode = HEXA_KE_FP64;
+    o->helm_element = HEXA_HELM_BUG_COMPAT;
+    o->precision = HEXA_PREC_FP64;
+    o->matvec_kernel = HEXA_MATVEC_AUTO;
+    o->mg_levels = 0;
+    o->mg_pre = 2;
+    o->mg_post = 2;
+    o->mg_omega = 0.6;
+    o->nrhs = 1;
+    o->diag_mode = HEXA_DIAG_EXACT;
+}
+
+void hexa_default_solve_params(hexa_solve_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->tol = 1e-8;
+    p->max_iter = 30000;
+    p->shift_factor = 1e-10;
+    p->max_shift_attempts = 3;
+    p->shift_multiplier = 10.0;
+    p->stagnation_tol = 0.0;
+    p->precond = HEXA_PRECOND_JACOBI;
+    p->semantics = HEXA_SEMANTICS_GPU;
+    p->warm_start = 0;
+    p->check_every = 0;
+}
+
+int32_t hexa_device_info(char* name, int32_t len, int64_t* total_bytes, int32_t* n_cu) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error("no HIP device: %s", hipGetErrorString(e));
+        return HEXA_E_DEVICE;
+    }
+    int dev = 0;
+    HX_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HX_CHECK(hipGetDeviceProperties(&p, dev));
+    if (name && len > 0) {
+        std::snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (total_bytes) *total_bytes = (int64_t)p.totalGlobalMem;
+    if (n_cu) *n_cu = p.multiProcessorCount;
+    return HEXA_OK;
+}
+
+int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle** out) {
+    if (!grid || !out) { set_error("null argument"); return HEXA_E_INVALID; }
+    *out = nullptr;
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || !(grid->dx > 0) || !(grid->dy > 0) || !(grid->dz > 0)) {
+        set_error("invalid grid %d x %d x %d (h = %g, %g, %g)", grid->nx, grid->ny, grid->nz, grid->dx,
+                  grid->dy, grid->dz);
+        return HEXA_E_INVALID;
+    }
+    const int64_t nn = (int64_t)(grid->nx + 1) * (grid->ny + 1) * (grid->nz + 1);
+    if (3 * nn > ((int64_t)1 << 40)) { set_error("grid too large"); return HEXA_E_INVALID; }
+    hexa_options o;
+    if (opt) o = *opt; else hexa_default_options(&o);
+    if (o.nrhs < 1 || o.nrhs > 64) { set_error("nrhs must be in [1, 64]"); return HEXA_E_INVALID; }
+    if (!(o.mg_omega > 0.0 && o.mg_omega < 2.0)) { set_error("mg_omega must be in (0, 2)"); return HEXA_E_INVALID; }
+    if (o.mg_pre < 1 || o.mg_post < 0 || o.mg_pre > 16 || o.mg_post > 16) {
+        set_error("mg_pre must be in [1,16], mg_post in [0,16]");
+        return HEXA_E_INVALID;
+    }
+    if (!(o.penal > 0) || o.E0 < o.Emin) { set_error("invalid material law"); return HEXA_E_INVALID; }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) { set_error("no HIP device available"); return HEXA_E_DEVICE; }
+    hexa_handle* h = new hexa_handle();
+    h->grid = *grid;
+    h->opt = o;
+    h->g = make_grid(grid->nx, grid->ny, grid->nz);
+    if (!element_stiffness(grid->dx, grid->dy, grid->dz, o.nu, o.ke_mode, h->Ke)) {
+        delete h;
+        set_error("invalid element (nu must be in (-1, 0.5))");
+        return HEXA_E_INVALID;
+    }
+    auto fail = [&](int32_t s) { hexa_destroy(h); return s; };
+    if (hipGetDevice(&h->device) != hipSuccess) return fail(HEXA_E_DEVICE);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(HEXA_E_DEVICE);
+    if (hipEventCreate(&h->ev_a) != hipSuccess || hipEventCreate(&h->ev_b) != hipSuccess) return fail(HEXA_E_DEVICE);
+    for (int i = 0; i < RING; ++i)
+        if (hipEventCreateWithFlags(&h->ring_ev[i], hipEventDisableTiming) != hipSuccess) return fail(HEXA_E_DEVICE);
+    if (hipHostMalloc((void**)&h->h_state, sizeof(PcgState) * RING, hipHostMallocDefault) != hipSuccess)
+        return fail(HEXA_E_DEVICE);
+    int32_t s = dalloc(h, &h->d_Ke64, 576);
+    if (s != HEXA_OK) return fail(s);
+    if (hipMemcpy(h->d_Ke64, h->Ke, sizeof h->Ke, hipMemcpyHostToDevice) != hipSuccess) return fail(HEXA_E_DEVICE);
+    *out = h;
+    return HEXA_OK;
+}
+
+int32_t hexa_destroy(hexa_handle* h) {
+    if (!h) return HEXA_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) hipFree(p);
+    if (h->h_state) hipHostFree(h->h_state);
+    for (int i = 0; i < RING; ++i)
+        if (h->ring_ev[i]) hipEventDestroy(h->ring_ev[i]);
+    if (h->ev_a) hipEventDestroy(h->ev_a);
+    if (h->ev_b) hipEventDestroy(h->ev_b);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return HEXA_OK;
+}
+
+int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double* f,
+                         const uint8_t* active_elem) {
+    if (!h || !f) { set_error("null argument"); return HEXA_E_INVALID; }
+    const int64_t nd = 3 * h->g.nn;
+    HX_TRY(dalloc(h, &h->d_mask, (size_t)h->g.nn));
+    HX_TRY(dalloc(h, &h->d_f, (size_t)(nd * h->opt.nrhs)));
+    HX_TRY(dalloc(h, &h->d_fixed, (size_t)nd));
+    hipStream_t st = h->stream;
+    if (fixed_dof) HX_CHECK(hipMemcpyAsync(h->d_fixed, fixed_dof, (size_t)nd, hipMemcpyHostToDevice, st));
+    else launch_fill<uint8_t>(st, nd, 0, h->d_fixed);
+    if (active_elem) {
+        HX_TRY(dalloc(h, &h->d_active, (size_t)h->g.ne));
+        HX_CHECK(hipMemcpyAsync(h->d_active, active_elem, (size_t)h->g.ne, hipMemcpyHostToDevice, st));
+    }
+    const uint8_t* act = active_elem ? h->d_active : nullptr;
+    launch_node_mask(st, h->g, h->d_fixed, act, h->d_mask);
+    HX_CHECK(hipMemcpyAsync(h->d_f, f, sizeof(double) * nd * h->opt.nrhs, hipMemcpyHostToDevice, st));
+    HX_TRY(check_launch());
+    HX_CHECK(hipStreamSynchronize(st));
+    h->have_problem = true;
+    h->precond_ready = -1;
+    h->have_density = false;  // the factor depends on the active set
+    return HEXA_OK;
+}
+
+static int32_t finish_density(hexa_handle* h) {
+    hipStream_t st = h->stream;
+    HX_TRY(dalloc(h, &h->d_fac, (size_t)h->g.ne));
+    const uint8_t* act = (h->have_problem && h->d_active) ? h->d_active : nullptr;
+    launch_stiffness_factor<double>(st, h->g.ne, h->d_rho, act, h->opt.E0, h->opt.Emin, h->opt.penal,
+                                    h->d_fac);
+    HX_TRY(check_launch());
+    HX_CHECK(hipStreamSynchronize(st));
+    h->have_density = true;
+    h->fac32_valid = false;
+    h->precond_ready = -1;
+    return HEXA_OK;
+}
+
+int32_t hexa_set_density(hexa_handle* h, const double* rho) {
+    if (!h || !rho) { set_error("null argument"); return HEXA_E_INVALID; }
+    HX_TRY(dalloc(h, &h->d_rho, (size_t)h->g.ne));
+    HX_CHECK(hipMemcpyAsync(h->d_rho, rho, sizeof(double) * h->g.ne, hipMemcpyHostToDevice, h->stream));
+    return finish_density(h);
+}
+
+int32_t hexa_set_density_dev(hexa_handle* h, const double* d_rho) {
+    if (!h || !d_rho) { set_error("null argument"); return HEXA_E_INVALID; }
+    HX_TRY(dalloc(h, &h->d_rho, (size_t)h->g.ne));
+    HX_CHECK(hipMemcpyAsync(h->d_rho, d_rho, sizeof(double) * h->g.ne, hipMemcpyDeviceToDevice, h->stream));
+    return finish_density(h);
+}
+
+int32_t hexa_solve(hexa_handle* h, double* u_inout, const hexa_solve_params* prm, hexa_solve_report* rep) {
+    HX_TRY(need_ready(h));
+    hexa_solve_params p;
+    if (prm) p = *prm; else hexa_default_solve_params(&p);
+    if (!(p.tol > 0) || p.max_iter < 1 || p.shift_factor < 0 || !(p.shift_multiplier > 0)) {
+        set_error("invalid solve parameters");
+        return HEXA_E_INVALID;
+    }
+    HX_TRY(ensure_solver_vectors(h));
+    const int64_t nd = 3 * h->g.nn;
+    int32_t worst = HEXA_OK;
+    for (int r = 0; r < h->opt.nrhs; ++r) {
+        int32_t s = solve_one(h, r, u_inout ? u_inout + r * nd : nullptr, &p, rep ? rep + r : nullptr);
+        if (s == HEXA_E_NAN_INPUT || s == HEXA_E_OOM || s == HEXA_E_DEVICE || s == HEXA_E_INVALID) return s;
+        if (s != HEXA_OK) worst = s;
+    }
+    return worst;
+}
+
+int32_t hexa_get_displacement(hexa_handle* h, double* u) {
+    if (!h || !u || !h->d_x) { set_error("no solution available"); return HEXA_E_INVALID; }
+    HX_CHECK(hipMemcpyAsync(u, h->d_x, sizeof(double) * 3 * h->g.nn, hipMemcpyDeviceToHost, h->stream));
+    HX_CHECK(hipStreamSynchronize(h->stream));
+    return HEXA_OK;
+}
+
+int32_t hexa_displacement_dev(hexa_handle* h, const double** d_u) {
+    if (!h || !d_u || !h->d_x) { set_error("no solution available"); return HEXA_E_INVALID; }
+    *d_u = h->d_x;
+    return HEXA_OK;
+}
+
+int32_t hexa_matvec(hexa_handle* h, const double* x, double* y) {
+    if (!h || !x || !y) { set_error("null argument"); return HEXA_E_INVALID; }
+    if (!h->have_density) { set_error("hexa_set_density has not been called"); return HEXA_E_INVALID; }
+    const int64_t nd = 3 * h->g.nn;
+    HX_TRY(dalloc(h, &h->d_p, (size_t)nd));
+    HX_TRY(dalloc(h, &h->d_Ap, (size_t)nd));
+    hipStream_t st = h->stream;
+    HX_CHECK(hipMemcpyAsync(h->d_p, x, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    launch_elastic_op<double, double>(st, h->g, OP_APPLY, h->d_Ke64, h->d_fac, nullptr, h->d_p, nullptr,
+                                      nullptr, 0.0, 0.0, h->d_Ap);
+    HX_TRY(check_launch());
+    HX_CHECK(hipMemcpyAsync(y, h->d_Ap, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    return HEXA_OK;
+}
+
+int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t precision, int32_t reps) {
+    if (!h || !d_x || !d_y || reps < 1) { set_error("invalid argument"); return HEXA_E_INVALID; }
+    if (!h->have_density) { set_error("hexa_set_density has not been called"); return HEXA_E_INVALID; }
+    hipStream_t st = h->stream;
+    if (precision == HEXA_PREC_MIXED) HX_TRY(ensure_fac32(h));
+    HX_CHECK(hipEventRecord(h->ev_a, st));
+    for (int r = 0; r < reps; ++r) {
+        if (precision == HEXA_PREC_MIXED)
+            launch_elastic_op<float, float>(st, h->g, OP_APPLY, h->d_Ke32, h->d_fac32, nullptr,
+                                            (const float*)d_x, nullptr, nullptr, 0.0f, 0.0f, (float*)d_y);
+        else
+            launch_elastic_op<double, double>(st, h->g, OP_APPLY, h->d_Ke64, h->d_fac, nullptr,
+                                              (const double*)d_x, nullptr, nullptr, 0.0, 0.0, (double*)d_y);
+    }
+    HX_CHECK(hipEventRecord(h->ev_b, st));
+    HX_TRY(check_launch());
+    HX_CHECK(hipEventSynchronize(h->ev_b));
+    float ms = 0.f;
+    HX_CHECK(hipEventElapsedTime(&ms, h->ev_a, h->ev_b));
+    h->last_kernel_ms = (double)ms / reps;
+    return HEXA_OK;
+}
+
+int32_t hexa_apply_precond(hexa_handle* h, const double* r, double* z) {
+    if (!h || !r || !z) { set_error("null argument"); return HEXA_E_INVALID; }
+    if (h->precond_ready != HEXA_PRECOND_MG) { set_error("no multigrid set-up: solve with HEXA_PRECOND_MG first"); return HEXA_E_INVALID; }
+    const int64_t nd = 3 * h->g.nn;
+    hipStream_t st = h->stream;
+    HX_CHECK(hipMemcpyAsync(h->d_r, r, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+    apply_mg(h, h->d_r, h->d_z);
+    HX_TRY(check_launch());
+    HX_CHECK(hipMemcpyAsync(z, h->d_z, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    return HEXA_OK;
+}
+
+int32_t hexa_last_kernel_ms(hexa_handle* h, double* ms) {
+    if (!h || !ms) { set_error("null argument"); return HEXA_E_INVALID; }
+    *ms = h->last_kernel_ms;
+    return HEXA_OK;
+}
+
+int32_t hexa_stream_sync(hexa_handle* h) {
+    if (!h) { set_error("null handle"); return HEXA_E_INVALID; }
+    HX_CHECK(hipStreamSynchronize(h->stream));
+    return HEXA_OK;
+}
+
+}  // extern "C"

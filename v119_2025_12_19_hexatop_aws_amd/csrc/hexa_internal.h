@@ -1,0 +1,162 @@
+// Internal declarations shared by the libhexa translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/hexa.h"
+
+namespace hexa {
+
+void set_error(const char* fmt, ...);
+
+#define HX_CHECK(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::hexa::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                              __FILE__, __LINE__);                                      \
+            return e_ == hipErrorOutOfMemory ? HEXA_E_OOM : HEXA_E_DEVICE;              \
+        }                                                                               \
+    } while (0)
+
+#define HX_TRY(expr)                      \
+    do {                                  \
+        int32_t s_ = (expr);              \
+        if (s_ != HEXA_OK) return s_;     \
+    } while (0)
+
+// Reductions: every reducing kernel writes one partial per block into a
+// [quantity][RED_BLOCKS] array; a single-block kernel then sums them in a
+// fixed order, so every reduction is deterministic run to run and
+// independent of what else shares the GPU.
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+constexpr int MAX_RED_Q = 4;
+
+// Node mask bits.  Bits 0..2: component held.  Bit 3: node excluded (no
+// active element touches it; all three components are then held).
+constexpr uint8_t MASK_ALL = 0x7;
+
+// Operator application modes of the fused elasticity kernel.
+enum OpMode { OP_APPLY = 0, OP_RESID = 1, OP_SMOOTH = 2 };
+
+// Grid description of one level.
+struct Grid3 {
+    int nx, ny, nz;     // elements
+    int64_t nn;         // nodes
+    int64_t ne;         // elements
+};
+
+inline Grid3 make_grid(int nx, int ny, int nz) {
+    Grid3 g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    g.ne = (int64_t)nx * ny * nz;
+    return g;
+}
+
+// Factored element operator (exact brick stiffness).  In the 2x2x2
+// Hadamard ("modal") basis of each displacement component the hex8 brick
+// stiffness is almost diagonal: per component i a diagonal over the 8
+// modes, plus, for each ordered component pair (i,j), two couplings of two
+// modes each.  All coefficients are scaled by the level's size.
+struct FactoredOp {
+    double diag[3][8];   // modal diagonal per output component
+    double lam[3];       // lambda * Mz-type mass for the (x,y),(y,z),(x,z) couplings, per mode parity
+    double mu[3];
+    double cpl[3][2][2]; // [pair][which][m] coupling coefficient
+};
+
+// Device-side state of one PCG recurrence.  The iteration's scalars live on
+// the device, so the host never waits inside an iteration; it reads this
+// struct back only to decide when to stop.
+enum PcgHalt { PCG_RUNNING = 0, PCG_CONVERGED = 1, PCG_BREAKDOWN = 2, PCG_NAN = 3, PCG_DIVERGED = 4 };
+struct PcgState {
+    double bnorm;      // ||b||
+    double rz;         // (r, z) of the current direction
+    double pAp;
+    double alpha, beta;
+    double rr;         // ||r||^2 after the last update
+    double rel;        // last checked ||r||/||b||
+    double best_rel;
+    double final_rel;
+    double zAp;
+    double tol;
+    int32_t halted;    // PcgHalt
+    int32_t improved;  // last check improved best_rel
+    int32_t last_k;    // last iteration that updated x
+    int32_t pad;
+};
+
+// ---- kernel launchers (implemented in the .hip files) -----------------
+// Elasticity (k_elastic.hip)
+template <typename T, typename DT>
+void launch_elastic_op(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, const T* fac,
+                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega,
+                       T shift, T* y);
+template <typename T>
+void launch_stiffness_factor(hipStream_t s, int64_t ne, const double* rho, const uint8_t* active,
+                             double E0, double Emin, double penal, T* fac);
+template <typename T>
+void launch_diag(hipStream_t s, const Grid3& g, const double* Ke, const T* fac, const uint8_t* mask,
+                 int x_entry, T* diag);
+void launch_node_mask(hipStream_t s, const Grid3& g, const uint8_t* fixed_dof, const uint8_t* active,
+                      uint8_t* mask);
+template <typename T>
+void launch_coarsen_factor(hipStream_t s, const Grid3& fine, const T* ffac, T* cfac);
+void launch_coarsen_mask(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
+                         uint8_t* cmask);
+
+// Multigrid transfers (k_mg.hip)
+template <typename T>
+void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
+                     const uint8_t* cmask, T* rc);
+template <typename T>
+void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
+                        const uint8_t* fmask, T* xf);
+void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,
+                            const uint8_t* mask, double shift, double* A);
+void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf);
+template <typename T>
+void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y);
+
+// Vectors and reductions (k_vector.hip)
+template <typename T>
+void launch_fill(hipStream_t s, int64_t n, T v, T* x);
+template <typename A, typename B>
+void launch_convert(hipStream_t s, int64_t n, const A* x, B* y);
+template <typename T, typename DT>
+void launch_scale_dinv(hipStream_t s, int64_t n, const T* b, const DT* dinv, T omega, T* x);
+template <typename DT>
+void launch_make_dinv(hipStream_t s, int64_t n, const double* diag, const uint8_t* mask, double shift,
+                      int clamp_mode, DT* dinv);
+void launch_dot2(hipStream_t s, int64_t n, const double* a, const double* b, double* partials);
+void launch_max_free(hipStream_t s, const Grid3& g, const double* diag, const uint8_t* mask,
+                     double* partials, double* out);
+void launch_final_sum(hipStream_t s, int nq, const double* partials, double* out);
+template <typename T>
+void launch_axpy(hipStream_t s, int64_t n, T a, const T* x, T* y);
+
+// PCG (k_pcg.hip).  All take the device state and do nothing once it halted.
+void launch_pcg_init_state(hipStream_t s, PcgState* st, const double* partials, double tol);
+void launch_pcg_rz(hipStream_t s, int64_t n, const double* r, const double* z, double* partials);
+void launch_pcg_set_rz(hipStream_t s, PcgState* st, const double* partials);
+void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, const double* Ap,
+                    double* partials);
+void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check);
+void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
+                       const double* Ap, const double* minv, double* partials);
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
+                      int cpu_semantics, int jacobi);
+void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
+                          const double* Ap, double* partials);
+void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials);
+void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
+                         const double* r, const double* minv);
+void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best);
+
+}  // namespace hexa

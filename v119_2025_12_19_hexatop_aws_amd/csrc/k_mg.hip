@@ -1,0 +1,219 @@
+// Geometric multigrid kernels: structured 2x trilinear transfers, the dense
+// coarsest-level operator and its inverse.
+//
+// The reference's "multigrid" (R/src/Solvers/GPUGeometricMultigrid.jl:57-209)
+// transfers residuals between a fine and one coarse grid through trilinear
+// weights computed from node coordinates, with 24 atomics per fine node in
+// the restriction.  On the structured grid the same trilinear operator P is
+// the 2x tensor-product stencil (weights 1, 1/2, 1/4, 1/8), so both P and
+// R = P^T are written here as gathers: no coordinates, no atomics, and R is
+// the exact adjoint of P.
+#include "hexa_internal.h"
+
+namespace hexa {
+namespace {
+
+inline unsigned grid_for(int64_t n, int threads = 256) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b > 65535 * 8) b = 65535 * 8;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+// r_c(I,J,K) = sum over the 3x3x3 fine nodes around (2I,2J,2K) of
+// w(di) w(dj) w(dk) r_f, w(0) = 1, w(+-1) = 1/2; held coarse DOFs -> 0.
+template <typename T>
+__global__ void k_restrict(Grid3 f, Grid3 c, const T* __restrict__ rf, const uint8_t* __restrict__ cmask,
+                           T* __restrict__ rc) {
+    const int CNX = c.nx + 1, CNY = c.ny + 1;
+    const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < c.nn;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(n % CNX), J = (int)((n / CNX) % CNY), K = (int)(n / ((int64_t)CNX * CNY));
+        const uint8_t mb = cmask[n];
+        T s0 = 0, s1 = 0, s2 = 0;
+        for (int dk = -1; dk <= 1; ++dk) {
+            const int k = 2 * K + dk;
+            if ((unsigned)k >= (unsigned)FNZ) continue;
+            const T wk = dk == 0 ? T(1) : T(0.5);
+            for (int dj = -1; dj <= 1; ++dj) {
+                const int j = 2 * J + dj;
+                if ((unsigned)j >= (unsigned)FNY) continue;
+                const T wjk = wk * (dj == 0 ? T(1) : T(0.5));
+                for (int di = -1; di <= 1; ++di) {
+                    const int i = 2 * I + di;
+                    if ((unsigned)i >= (unsigned)FNX) continue;
+                    const T w = wjk * (di == 0 ? T(1) : T(0.5));
+                    const int64_t m = i + (int64_t)FNX * (j + (int64_t)FNY * k);
+                    s0 += w * rf[3 * m];
+                    s1 += w * rf[3 * m + 1];
+                    s2 += w * rf[3 * m + 2];
+                }
+            }
+        }
+        rc[3 * n] = (mb & 1) ? T(0) : s0;
+        rc[3 * n + 1] = (mb & 2) ? T(0) : s1;
+        rc[3 * n + 2] = (mb & 4) ? T(0) : s2;
+    }
+}
+
+// x_f += P x_c, masked by the fine supports.  Along each axis an even fine
+// index coincides with coarse node i/2 (weight 1); an odd one lies midway
+// between (i-1)/2 and (i+1)/2 (weights 1/2).
+template <typename T>
+__global__ void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
+                              const uint8_t* __restrict__ fmask, T* __restrict__ xf) {
+    const int CNX = c.nx + 1, CNY = c.ny + 1;
+    const int FNX = f.nx + 1, FNY = f.ny + 1;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < f.nn;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(n % FNX), j = (int)((n / FNX) % FNY), k = (int)(n / ((int64_t)FNX * FNY));
+        const uint8_t mb = fmask[n];
+        if ((mb & 7) == 7) continue;
+        const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+        const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+        const T wi = ni == 2 ? T(0.5) : T(1), wj = nj == 2 ? T(0.5) : T(1), wk = nk == 2 ? T(0.5) : T(1);
+        T s0 = 0, s1 = 0, s2 = 0;
+        for (int a = 0; a < nk; ++a)
+            for (int bb = 0; bb < nj; ++bb)
+                for (int cc = 0; cc < ni; ++cc) {
+                    const int64_t m = (i0 + cc) + (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
+                    s0 += xc[3 * m];
+                    s1 += xc[3 * m + 1];
+                    s2 += xc[3 * m + 2];
+                }
+        const T w = wi * wj * wk;
+        if (!(mb & 1)) xf[3 * n] += w * s0;
+        if (!(mb & 2)) xf[3 * n + 1] += w * s1;
+        if (!(mb & 4)) xf[3 * n + 2] += w * s2;
+    }
+}
+
+__device__ constexpr int MCX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__device__ constexpr int MCY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__device__ constexpr int MCZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+__device__ __forceinline__ int corner_id(int cx, int cy, int cz) {
+    return 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0));
+}
+
+// Dense coarsest-level matrix (A zero-filled beforehand), one row per
+// thread (deterministic): row DOF
+// 3n+c sums factor_e * Ke(l,:) over the elements around node n.  Held DOFs
+// become identity rows/columns; the shift is added on the free diagonal.
+__global__ void k_coarse_assemble(Grid3 g, const double* __restrict__ Ke, const double* __restrict__ fac,
+                                  const uint8_t* __restrict__ mask, double shift, double* __restrict__ A) {
+    const int NX = g.nx + 1, NY = g.ny + 1;
+    const int64_t nd = 3 * g.nn;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < nd;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = row / 3;
+        const int c = (int)(row % 3);
+        double* Ar = A + row * nd;  // zero-filled by the caller
+        const bool held = (mask[n] >> c) & 1;
+        if (held) { Ar[row] = 1.0; continue; }
+        const int i = (int)(n % NX), j = (int)((n / NX) % NY), k = (int)(n / ((int64_t)NX * NY));
+        for (int s = 0; s < 8; ++s) {
+            const int ex = (s & 1) - 1, ey = ((s >> 1) & 1) - 1, ez = ((s >> 2) & 1) - 1;
+            const int ei = i + ex, ej = j + ey, ek = k + ez;
+            if ((unsigned)ei >= (unsigned)g.nx || (unsigned)ej >= (unsigned)g.ny ||
+                (unsigned)ek >= (unsigned)g.nz)
+                continue;
+            const double f = fac[ei + (int64_t)g.nx * (ej + (int64_t)g.ny * ek)];
+            const int l = corner_id(-ex, -ey, -ez);
+            for (int m = 0; m < 8; ++m) {
+                const int64_t nm = (ei + MCX[m]) + (int64_t)NX * ((ej + MCY[m]) + (int64_t)NY * (ek + MCZ[m]));
+                const uint8_t mm = mask[nm];
+                for (int cc = 0; cc < 3; ++cc) {
+                    if ((mm >> cc) & 1) continue;
+                    Ar[3 * nm + cc] += f * Ke[(3 * l + c) * 24 + 3 * m + cc];
+                }
+            }
+        }
+        Ar[row] += shift;
+    }
+}
+
+// In-place Gauss-Jordan inversion of an SPD matrix, one pivot per pair of
+// launches (no pivoting needed for SPD): save pivot row/column, then update.
+__global__ void k_gj_save(int64_t n, int64_t k, const double* __restrict__ A, double* __restrict__ rowb,
+                          double* __restrict__ colb) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        rowb[t] = A[k * n + t];
+        colb[t] = A[t * n + k];
+    }
+}
+
+__global__ void k_gj_update(int64_t n, int64_t k, double* __restrict__ A, const double* __restrict__ rowb,
+                            const double* __restrict__ colb) {
+    const double piv = 1.0 / rowb[k];
+    const int64_t nn = n * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t / n, j = t % n;
+        double v;
+        if (i == k && j == k) v = piv;
+        else if (i == k) v = rowb[j] * piv;
+        else if (j == k) v = -colb[i] * piv;
+        else v = A[t] - colb[i] * rowb[j] * piv;
+        A[t] = v;
+    }
+}
+
+// y = Ainv x with FP64 arithmetic, one wave per row (deterministic tree).
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __restrict__ Ainv,
+                                                    const T* __restrict__ x, T* __restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double* Ar = Ainv + row * n;
+    double s = 0.0;
+    for (int64_t c = lane; c < n; c += 64) s += Ar[c] * (double)x[c];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[row] = (T)s;
+}
+
+}  // namespace
+
+template <typename T>
+void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
+                     const uint8_t* cmask, T* rc) {
+    k_restrict<T><<<grid_for(coarse.nn), 256, 0, s>>>(fine, coarse, rf, cmask, rc);
+}
+
+template <typename T>
+void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
+                        const uint8_t* fmask, T* xf) {
+    k_prolong_add<T><<<grid_for(fine.nn), 256, 0, s>>>(fine, coarse, xc, fmask, xf);
+}
+
+void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,
+                            const uint8_t* mask, double shift, double* A) {
+    k_coarse_assemble<<<grid_for(3 * g.nn, 64), 64, 0, s>>>(g, Ke, fac, mask, shift, A);
+}
+
+void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf) {
+    const unsigned gs = grid_for(n), gu = grid_for(n * n);
+    for (int64_t k = 0; k < n; ++k) {
+        k_gj_save<<<gs, 256, 0, s>>>(n, k, A, rowbuf, colbuf);
+        k_gj_update<<<gu, 256, 0, s>>>(n, k, A, rowbuf, colbuf);
+    }
+}
+
+template <typename T>
+void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y) {
+    k_dense_gemv<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, Ainv, x, y);
+}
+
+template void launch_restrict<double>(hipStream_t, const Grid3&, const Grid3&, const double*,
+                                      const uint8_t*, double*);
+template void launch_restrict<float>(hipStream_t, const Grid3&, const Grid3&, const float*,
+                                     const uint8_t*, float*);
+template void launch_prolong_add<double>(hipStream_t, const Grid3&, const Grid3&, const double*,
+                                         const uint8_t*, double*);
+template void launch_prolong_add<float>(hipStream_t, const Grid3&, const Grid3&, const float*,
+                                        const uint8_t*, float*);
+template void launch_dense_gemv<double>(hipStream_t, int64_t, const double*, const double*, double*);
+template void launch_dense_gemv<float>(hipStream_t, int64_t, const double*, const float*, float*);
+
+}  // namespace hexa

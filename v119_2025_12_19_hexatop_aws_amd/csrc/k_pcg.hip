@@ -1,0 +1,263 @@
+// PCG recurrence kernels with device-resident scalars.
+//
+// The reference driver (R/src/Solvers/GPUSolver.jl:335-444) brings every
+// dot product and norm back to the host (:365,373,380,414) and snapshots
+// the best iterate to host memory (:384).  Here alpha, beta, the residual
+// check, the divergence tests and the best-iterate snapshot are evaluated on
+// the device by single-block kernels reading fixed-order partial sums; the
+// host only reads the state back to decide when to stop.  Every kernel is a
+// no-op once the state has halted, so iterations queued ahead of the host's
+// check cost nothing but their launch.
+#include "hexa_internal.h"
+
+namespace hexa {
+namespace {
+
+inline unsigned grid_for(int64_t n, int threads = 256) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b > 65535 * 8) b = 65535 * 8;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < RED_THREADS / 64; ++i) t += lds[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ double final_sum(const double* __restrict__ partials, int q, double* lds) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS) s += partials[q * RED_BLOCKS + i];
+    return block_sum(s, lds);
+}
+
+// partials[0] holds ||b||^2 on entry.
+__global__ __launch_bounds__(RED_THREADS) void k_init_state(PcgState* st, const double* partials,
+                                                            double tol) {
+    __shared__ double lds[RED_THREADS / 64];
+    const double bb = final_sum(partials, 0, lds);
+    if (threadIdx.x == 0) {
+        st->bnorm = sqrt(bb);
+        st->rz = st->pAp = st->alpha = st->beta = st->rr = st->zAp = 0.0;
+        st->rel = st->final_rel = 0.0;
+        st->best_rel = INFINITY;
+        st->tol = tol;
+        st->halted = PCG_RUNNING;
+        st->improved = 0;
+        st->last_k = 0;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_rz(int64_t n, const double* __restrict__ r,
+                                                    const double* __restrict__ z,
+                                                    double* __restrict__ partials) {
+    __shared__ double lds[RED_THREADS / 64];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
+        s += r[i] * z[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_set_rz(PcgState* st, const double* partials) {
+    __shared__ double lds[RED_THREADS / 64];
+    const double v = final_sum(partials, 0, lds);
+    if (threadIdx.x == 0) st->rz = v;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_pAp(int64_t n, const PcgState* st,
+                                                     const double* __restrict__ p,
+                                                     const double* __restrict__ Ap,
+                                                     double* __restrict__ partials) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS)
+        s += p[i] * Ap[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// alpha = rz / (p, Ap); a breakdown |pAp| < 1e-20 stops the attempt
+// (GPUSolver.jl:373-375).
+__global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const double* partials,
+                                                       int breakdown_check) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double den = final_sum(partials, 0, lds);
+    if (threadIdx.x == 0) {
+        st->pAp = den;
+        if (breakdown_check && fabs(den) < 1e-20) {
+            st->halted = PCG_BREAKDOWN;
+            st->alpha = 0.0;
+        } else {
+            st->alpha = st->rz / den;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha Ap; partials: (r,r) and, when minv is given,
+// (r, minv r) for the Jacobi preconditioner.
+__global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, const PcgState* st, double* __restrict__ x,
+                                                        const double* __restrict__ p, double* __restrict__ r,
+                                                        const double* __restrict__ Ap,
+                                                        const double* __restrict__ minv,
+                                                        double* __restrict__ partials) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double a = st->alpha;
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        rr += ri * ri;
+        if (minv) rz += ri * (ri * minv[i]);
+    }
+    rr = block_sum(rr, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = rr;
+    if (minv) {
+        rz = block_sum(rz, lds);
+        if (threadIdx.x == 0) partials[RED_BLOCKS + blockIdx.x] = rz;
+    }
+}
+
+// Residual check after the update.  GPU semantics (GPUSolver.jl:379-401),
+// evaluated when 'check' is set (k == 1 or k % 10 == 0 in the reference):
+// best-iterate tracking, rel < tol -> converged, NaN/Inf -> halt,
+// rel > 10 or (k > 500 and rel > 100 best) -> diverged.  CPU semantics
+// (CPUSolver.jl:233-239): rel < tol every iteration, nothing else.
+// For the Jacobi preconditioner beta = (r,z)_new/(r,z) is formed here too.
+__global__ __launch_bounds__(RED_THREADS) void k_check(PcgState* st, const double* partials, int k,
+                                                       int check, int cpu_semantics, int jacobi) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double rr = final_sum(partials, 0, lds);
+    const double rzn = jacobi ? final_sum(partials, 1, lds) : 0.0;
+    if (threadIdx.x != 0) return;
+    st->rr = rr;
+    st->last_k = k;
+    st->improved = 0;
+    if (check) {
+        const double rel = sqrt(rr) / st->bnorm;
+        st->rel = rel;
+        st->final_rel = rel;
+        if (cpu_semantics) {
+            if (rel < st->tol) { st->halted = PCG_CONVERGED; return; }
+        } else {
+            if (rel < st->best_rel) { st->best_rel = rel; st->improved = 1; }
+            if (rel < st->tol) { st->halted = PCG_CONVERGED; return; }
+            if (isnan(rel) || isinf(rel)) { st->halted = PCG_NAN; return; }
+            if (rel > 10.0 || (k > 500 && rel > st->best_rel * 100.0)) { st->halted = PCG_DIVERGED; return; }
+        }
+    }
+    if (jacobi) {
+        st->beta = rzn / st->rz;
+        st->rz = rzn;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_flex_dots(int64_t n, const PcgState* st,
+                                                           const double* __restrict__ r,
+                                                           const double* __restrict__ z,
+                                                           const double* __restrict__ Ap,
+                                                           double* __restrict__ partials) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    double rz = 0.0, zap = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const double zi = z[i];
+        rz += r[i] * zi;
+        zap += zi * Ap[i];
+    }
+    rz = block_sum(rz, lds);
+    zap = block_sum(zap, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = rz;
+        partials[RED_BLOCKS + blockIdx.x] = zap;
+    }
+}
+
+// Flexible (Polak-Ribiere) beta for a preconditioner that is not exactly a
+// fixed SPD matrix (the mixed-precision V-cycle):
+// beta = (z_new, r_new - r_old)/(z_old, r_old) = -alpha (z_new, Ap)/(z_old, r_old).
+__global__ __launch_bounds__(RED_THREADS) void k_flex_beta(PcgState* st, const double* partials) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double rz = final_sum(partials, 0, lds);
+    const double zap = final_sum(partials, 1, lds);
+    if (threadIdx.x == 0) {
+        st->zAp = zap;
+        st->beta = -st->alpha * zap / st->rz;
+        st->rz = rz;
+    }
+}
+
+// p = z + beta p; with z == nullptr, z = minv * r (Jacobi, not stored).
+__global__ void k_p_update(int64_t n, const PcgState* st, double* __restrict__ p, const double* __restrict__ z,
+                           const double* __restrict__ r, const double* __restrict__ minv) {
+    if (st->halted) return;
+    const double b = st->beta;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double zi = z ? z[i] : r[i] * minv[i];
+        p[i] = zi + b * p[i];
+    }
+}
+
+__global__ void k_best_copy(int64_t n, const PcgState* st, const double* __restrict__ x,
+                            double* __restrict__ best) {
+    if (!st->improved) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        best[i] = x[i];
+}
+
+}  // namespace
+
+void launch_pcg_init_state(hipStream_t s, PcgState* st, const double* partials, double tol) {
+    k_init_state<<<1, RED_THREADS, 0, s>>>(st, partials, tol);
+}
+void launch_pcg_rz(hipStream_t s, int64_t n, const double* r, const double* z, double* partials) {
+    k_rz<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, r, z, partials);
+}
+void launch_pcg_set_rz(hipStream_t s, PcgState* st, const double* partials) {
+    k_set_rz<<<1, RED_THREADS, 0, s>>>(st, partials);
+}
+void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, const double* Ap,
+                    double* partials) {
+    k_pAp<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, p, Ap, partials);
+}
+void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check) {
+    k_alpha<<<1, RED_THREADS, 0, s>>>(st, partials, breakdown_check);
+}
+void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
+                       const double* Ap, const double* minv, double* partials) {
+    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials);
+}
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
+                      int cpu_semantics, int jacobi) {
+    k_check<<<1, RED_THREADS, 0, s>>>(st, partials, k, check, cpu_semantics, jacobi);
+}
+void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
+                          const double* Ap, double* partials) {
+    k_flex_dots<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials);
+}
+void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials) {
+    k_flex_beta<<<1, RED_THREADS, 0, s>>>(st, partials);
+}
+void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
+                         const double* r, const double* minv) {
+    k_p_update<<<grid_for(n), 256, 0, s>>>(n, st, p, z, r, minv);
+}
+void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best) {
+    k_best_copy<<<grid_for(n), 256, 0, s>>>(n, st, x, best);
+}
+
+}  // namespace hexa
