@@ -1,0 +1,193 @@
+"""GPU parity of the linear-solve path: libhexa (through the C ABI) against the
+CPU oracle and against sparse direct solves, on seeded inputs."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import cantilever_arrays, direct_solve
+from v119_2025_12_19_hexatop_aws_amd import hexa
+from v119_2025_12_19_hexatop_aws_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("ke_mode", [hexa.KE_FP64, hexa.KE_F32_WIDENED])
+@pytest.mark.parametrize("grid,h", [((4, 3, 2), (1.0, 1.0, 1.0)), ((7, 5, 3), (1.0, 2.0, 0.5)),
+                                    ((33, 9, 5), (1.0, 1.0, 1.0))])
+def test_matvec_matches_oracle(grid, h, ke_mode):
+    say("matvec", grid, h, ke_mode)
+    nx, ny, nz = grid
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.001, 1.0, nx * ny * nz)
+    x = rng.standard_normal(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, ke_mode=ke_mode) as s:
+        s.set_density(rho)
+        y = s.matvec(x)
+    Ke = oracle.ke_hex8(*h, nu=0.3, f32=(ke_mode == hexa.KE_F32_WIDENED))
+    ref = oracle.apply_stiffness(grid, Ke, oracle.stiffness_factor(rho, 1.0, 1e-9, 3.0), x)
+    tol = 1e-13 if ke_mode == hexa.KE_FP64 else 1e-6
+    assert np.abs(y - ref).max() <= tol * np.abs(ref).max()
+
+
+def _golden_problem(tag):
+    nx, ny, nz, seed = (int(v) for v in GOLD[f"solve_{tag}_grid"])
+    fixed, f, _ = cantilever_arrays(nx, ny, nz, seed)
+    return (nx, ny, nz), fixed, f, GOLD[f"solve_{tag}_rho"], GOLD[f"solve_{tag}_u"]
+
+
+@pytest.mark.parametrize("tag", ["a", "b"])
+@pytest.mark.parametrize("precond,precision", [(hexa.PRECOND_JACOBI, hexa.PREC_FP64),
+                                               (hexa.PRECOND_MG, hexa.PREC_FP64),
+                                               (hexa.PRECOND_MG, hexa.PREC_MIXED)])
+def test_solve_matches_golden_direct(tag, precond, precision):
+    say("golden solve", tag, precond, precision)
+    grid, fixed, f, rho, ug = _golden_problem(tag)
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        u, reps, st = s.solve(tol=1e-12, precond=precond, semantics=hexa.SEMANTICS_CPU, shift_factor=1e-10,
+                              max_iter=20000)
+    assert st == hexa.OK, reps[0]
+    assert abs(reps[0]["shift_used"] - GOLD[f"solve_{tag}_shift"][0]) <= 1e-12 * reps[0]["shift_used"]
+    assert rel(u[0], ug) < 1e-9
+    assert abs(reps[0]["compliance"] - GOLD[f"solve_{tag}_compliance"][0]) <= 1e-10 * abs(f @ ug)
+
+
+def test_gpu_semantics_match_oracle_control_flow():
+    say("gpu semantics")
+    grid, fixed, f, rho, ug = _golden_problem("b")
+    Ke = oracle.ke_hex8(1, 1, 1, 0.3)
+    fac = oracle.stiffness_factor(rho, 1.0, 1e-9, 3.0)
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, diag_mode=hexa.DIAG_X_ENTRY) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        for tol, max_iter, attempts in ((1e-8, 20000, 3), (1e-12, 20000, 3), (1e-12, 5, 2)):
+            u, reps, st = s.solve(tol=tol, max_iter=max_iter, max_shift_attempts=attempts)
+            uo, sto, info = oracle.pcg_gpu_semantics(grid, Ke, fac, fixed, f, tol=tol, max_iter=max_iter,
+                                                     max_attempts=attempts)
+            r = reps[0]
+            assert (st == hexa.OK) == (sto == 0)
+            assert r["attempt"] == info["attempt"] and r["method_name"] == "Jacobi_GPU"
+            assert abs(r["iterations"] - info["iterations"]) <= 10
+            assert abs(r["shift_used"] - info["shift"]) <= 1e-12 * info["shift"]
+            assert rel(u[0], uo) < (1e-6 if sto == 0 else 1e-2)
+
+
+def test_corrupt_input_and_not_converged():
+    say("corrupt input")
+    grid, fixed, f, rho, ug = _golden_problem("a")
+    bad = rho.copy()
+    bad[3] = np.nan
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0) as s:
+        s.set_problem(fixed, f)
+        s.set_density(bad)
+        u, reps, st = s.solve()
+        assert st == hexa.E_NAN_INPUT and reps[0]["method_name"] == "Failed_Corrupt_Input"
+        assert np.all(u == 0) and np.isnan(reps[0]["rel_res"])
+        s.set_density(rho)
+        u, reps, st = s.solve(tol=1e-12, max_iter=3, max_shift_attempts=1)
+        assert st == hexa.E_NOT_CONVERGED and reps[0]["converged"] == 0
+        assert reps[0]["rel_res"] > 1e-12 and np.linalg.norm(u) > 0
+
+
+def test_warm_start_reduces_iterations():
+    say("warm start")
+    grid, fixed, f, rho, ug = _golden_problem("b")
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        u, r0, st = s.solve(tol=1e-8)
+        guess = ug * (1 + 1e-4 * np.random.default_rng(9).standard_normal(ug.size))
+        u1, r1, st1 = s.solve(guess[None, :], tol=1e-8, warm_start=1)
+    assert st == st1 == hexa.OK and r1[0]["iterations"] < r0[0]["iterations"]
+
+
+@pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
+def test_cantilever_c1_parity(precision):
+    """60x20x20 cantilever (SIMP p=3, Emin=1e-9, rho0 = 0.3 +- 0.01): the
+    MG-preconditioned solve against the oracle's CPU Jacobi-PCG, both solved
+    to 1e-11: displacements within 1e-8, compliance within 1e-9 (FP64)."""
+    say("c1 parity", precision)
+    p = P.cantilever_c1()
+    Ke = oracle.ke_hex8(1, 1, 1, 0.3)
+    fac = oracle.stiffness_factor(p.rho, 1.0, 1e-9, 3.0)
+    uo, relo, it, sh = oracle.pcg_cpu(p.grid, Ke, fac, p.fixed, p.f[0], tol=1e-11, max_iter=50000,
+                                      shift_factor=1e-10)
+    assert relo < 1e-11
+    with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
+        s.set_problem(p.fixed, p.f[0])
+        s.set_density(p.rho)
+        u, reps, st = s.solve(tol=1e-11, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU)
+    r = reps[0]
+    assert st == hexa.OK and r["true_rel_res"] < 1e-10, r
+    assert rel(u[0], uo) < 1e-8
+    co = p.f[0] @ uo
+    assert abs(r["compliance"] - co) < 1e-9 * abs(co)
+    assert r["iterations"] < 100
+
+
+def test_mg_preconditioner_is_symmetric_positive():
+    say("mg symmetry")
+    grid, fixed, f, rho, ug = _golden_problem("b")
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        s.solve(tol=1e-6, precond=hexa.PRECOND_MG)
+        rng = np.random.default_rng(3)
+        free = np.asarray(fixed) == 0
+        a = rng.standard_normal(free.size) * free
+        b = rng.standard_normal(free.size) * free
+        Ma, Mb = s.apply_precond(a), s.apply_precond(b)
+    assert abs(Ma @ b - a @ Mb) < 1e-11 * abs(Ma @ b)
+    assert Ma @ a > 0 and Mb @ b > 0
+    assert np.all(Ma[~free] == 0)
+
+
+def test_solve_system_gpu_mirror_and_pruning():
+    """The reference-shaped entry points: generate_mesh-style node/element
+    arrays, bc_indicator, pruning of voids at outer iteration >= 3."""
+    say("mirror")
+    nx, ny, nz = 8, 4, 4
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    nodes = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.float32)
+    ek, ej, ei = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = (ei + (nx + 1) * (ej + (ny + 1) * ek)).ravel()
+    corners = [0, 1, 1 + (nx + 1), nx + 1]
+    off = corners + [c + (nx + 1) * (ny + 1) for c in corners]
+    elements = np.stack([base + o for o in off], 1) + 1
+    fixed, f, _ = cantilever_arrays(nx, ny, nz, 12)
+    bc = fixed.reshape(-1, 3).astype(np.float32)
+    rho = np.random.default_rng(2).uniform(0.2, 1.0, nx * ny * nz).astype(np.float32)
+    rho[[5, 6, 40, 41, 42]] = 0.001         # voids, pruned at iter >= 3
+    cfg = {"geometry": {"nElem_x_computed": nx, "nElem_y_computed": ny, "nElem_z_computed": nz,
+                        "dx_computed": 1.0, "dy_computed": 1.0, "dz_computed": 1.0},
+           "solver_parameters": {"tolerance": 1e-10, "max_iterations": 20000, "diagonal_shift_factor": 1e-10,
+                                 "preconditioner": "multigrid", "ke_mode": "fp64"},
+           "current_outer_iter": 5}
+    U, res, meth = hexa.solve_system(nodes, elements, 1.0, 0.3, bc, f, density=rho, config=cfg,
+                                     min_stiffness_threshold=0.001)
+    assert meth == "GMG_GPU" and res < 1e-10
+    # oracle: the pruned system = inactive elements with zero stiffness, orphan nodes held
+    act = rho > np.float32(0.001) * np.float32(1.01)
+    fac = np.where(act, rho.astype(np.float64), 0.0)
+    touched = np.zeros((nx + 1) * (ny + 1) * (nz + 1), bool)
+    touched[elements[act].ravel() - 1] = True
+    fx = fixed.reshape(-1, 3).copy()
+    fx[~touched] = 1
+    Ke = oracle.ke_hex8(1, 1, 1, 0.3)
+    uo, relo, it, sh = oracle.pcg_cpu((nx, ny, nz), Ke, fac, fx.ravel(), f, tol=1e-12, max_iter=50000,
+                                      shift_factor=float(np.float32(1e-10)))
+    assert rel(U, uo) < 1e-7
