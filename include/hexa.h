@@ -175,6 +175,87 @@ int32_t hexa_apply_precond(hexa_handle* h, const double* r, double* z);
 int32_t hexa_last_kernel_ms(hexa_handle* h, double* ms);
 int32_t hexa_stream_sync(hexa_handle* h);
 
+/* ---- Helmholtz density filter ------------------------------------------
+ * Replaces GPUHelmholtz.apply_gpu_filter! (GPUHelmholtz.jl:422-443) with the
+ * solve semantics of solve_helmholtz_on_gpu (:190-266): (R^2 K + M) x = b,
+ * b_n = sum rho_e V/8, Jacobi-PCG from zero, ||r||/||b|| checked every
+ * check_every iterations (reference: 10) up to max_iter (reference: 200),
+ * output rho~_e = mean of the element's 8 nodal values.  The element matrix
+ * follows opt->helm_element.  rho_in/rho_out: nE host arrays (may alias).
+ * A zero right-hand side returns rho_in unchanged with 0 iterations. */
+typedef struct {
+    int32_t iterations;
+    int32_t pad;
+    double rel_res;            /* last checked residual; 0 if none checked */
+    double ms;
+} hexa_filter_report;
+int32_t hexa_filter(hexa_handle* h, const double* rho_in, double* rho_out, double radius, double tol,
+                    int32_t max_iter, int32_t check_every, hexa_filter_report* rep);
+
+/* ---- element stress ------------------------------------------------------
+ * Replaces Stress.compute_stress_field (Stress.jl:287-374): centroid stress
+ * of every element with E_e = E * rho_e, von Mises, principal values
+ * (descending), l1 = sum |sigma_i|, directions of the largest/smallest;
+ * rho < 1e-6 -> zeros.  u: 3nN host array or NULL (the handle's last
+ * solution); rho: nE host array or NULL (the handle's density).  Outputs
+ * are Float32 like the reference (3nE, nE, nE, 3nE, 3nE, 6nE); any may be NULL. */
+int32_t hexa_stress(hexa_handle* h, const double* u, const double* rho, double E, float* principal,
+                    float* vonmises, float* l1, float* dir_max, float* dir_min, float* voigt);
+
+/* ---- SIMP sensitivity and OC update (no reference counterpart) ---------
+ * hexa_sensitivity: dc/drho~_e = -p rho~^(p-1) (E0-Emin) u_e' K0 u_e for the
+ * handle's density (the physical field) and last solution; also returns
+ * the compliance.  dc: nE host array. */
+int32_t hexa_sensitivity(hexa_handle* h, double* dc, double* compliance);
+/* Optimality-criteria update of x (nE, host, in/out):
+ * x_new = clamp(x sqrt(-dc/(lambda dv)), max(0,x-move), min(1,x+move)),
+ * lambda by multisection on [0, 1e9] until (l2-l1)/(l1+l2) <= bisect_tol
+ * such that sum w x_new = volfrac * nE.  Returns lambda in *lambda_out. */
+int32_t hexa_oc_update(hexa_handle* h, double* x, const double* dc, const double* dv, const double* w,
+                       double volfrac, double move, double bisect_tol, double* lambda_out);
+
+typedef struct {
+    double volfrac, move, rmin;   /* rmin = Helmholtz radius of the density filter */
+    double filter_tol, bisect_tol;
+    int32_t filter_max_iter;
+    int32_t pad;
+    hexa_solve_params solve;      /* warm_start uses the previous step's u */
+} hexa_oc_params;
+typedef struct {
+    double compliance, volume, change, lambda;
+    double solve_ms, step_ms;
+    int32_t solve_iterations, filter_iterations;
+    int32_t status;               /* status of the linear solve */
+    int32_t pad;
+} hexa_oc_report;
+/* One SIMP/OC iteration on the device: x~ = F(x) (PDE filter), solve
+ * K(x~) u = f, dc~, dc = F^T dc~, dv = F^T 1, OC update of the design x
+ * (nE, host, in/out).  The filter is linear and symmetric, so F^T = F and
+ * the volume of the filtered design is sum (F 1)_e x_e. */
+int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc_report* rep);
+
+/* ---- reference stress-ratio density update -------------------------------
+ * Replaces TopologyOptimization.update_density! (TopOpt.jl:211-482) on the
+ * handle's grid: proposed density from l1, radius schedule, Helmholtz
+ * filter with R/2.5, clamp, mean change, threshold schedule, culling of the
+ * lowest candidates in stable (density, index) order, optional "hard"
+ * projection, protected elements restored.  density (nE, Float32, in/out),
+ * l1 (nE), protected_mask (nE bytes), original_density (nE). */
+typedef struct {
+    double E, l1_allow, min_density, max_density, max_domain_dim, filter_tol;
+    double R_init_perc, R_interm_perc, R_final_perc, R_interm_iter_perc; /* percents */
+    double min_radius_factor, final_threshold, max_culling_ratio;
+    int32_t iter, n_iters, hard, pad;
+} hexa_update_params;
+typedef struct {
+    double mean_change, R_length, threshold, filter_res;
+    int32_t filter_iters, skipped;  /* skipped: NaN in l1 (TopOpt.jl:230-236) */
+    int64_t culled;
+} hexa_update_report;
+int32_t hexa_update_density(hexa_handle* h, float* density, const float* l1, const uint8_t* protected_mask,
+                            const float* original_density, const hexa_update_params* p,
+                            hexa_update_report* rep);
+
 #ifdef __cplusplus
 }
 #endif

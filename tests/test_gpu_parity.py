@@ -24,16 +24,18 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("ke_mode", [hexa.KE_FP64, hexa.KE_F32_WIDENED])
+@pytest.mark.parametrize("ke_mode,kernel", [(hexa.KE_FP64, hexa.MATVEC_FACTORED), (hexa.KE_FP64, hexa.MATVEC_DENSE),
+                                            (hexa.KE_F32_WIDENED, hexa.MATVEC_AUTO)])
 @pytest.mark.parametrize("grid,h", [((4, 3, 2), (1.0, 1.0, 1.0)), ((7, 5, 3), (1.0, 2.0, 0.5)),
-                                    ((33, 9, 5), (1.0, 1.0, 1.0))])
-def test_matvec_matches_oracle(grid, h, ke_mode):
-    say("matvec", grid, h, ke_mode)
+                                    ((33, 9, 5), (1.0, 1.0, 1.0)), ((15, 14, 40), (0.5, 1.0, 1.5)),
+                                    ((1, 1, 1), (1.0, 1.0, 1.0))])
+def test_matvec_matches_oracle(grid, h, ke_mode, kernel):
+    say("matvec", grid, h, ke_mode, kernel)
     nx, ny, nz = grid
     rng = np.random.default_rng(11)
     rho = rng.uniform(0.001, 1.0, nx * ny * nz)
     x = rng.standard_normal(3 * (nx + 1) * (ny + 1) * (nz + 1))
-    with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, ke_mode=ke_mode) as s:
+    with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, ke_mode=ke_mode, matvec_kernel=kernel) as s:
         s.set_density(rho)
         y = s.matvec(x)
     Ke = oracle.ke_hex8(*h, nu=0.3, f32=(ke_mode == hexa.KE_F32_WIDENED))
@@ -49,13 +51,14 @@ def _golden_problem(tag):
 
 
 @pytest.mark.parametrize("tag", ["a", "b"])
-@pytest.mark.parametrize("precond,precision", [(hexa.PRECOND_JACOBI, hexa.PREC_FP64),
-                                               (hexa.PRECOND_MG, hexa.PREC_FP64),
-                                               (hexa.PRECOND_MG, hexa.PREC_MIXED)])
-def test_solve_matches_golden_direct(tag, precond, precision):
-    say("golden solve", tag, precond, precision)
+@pytest.mark.parametrize("precond,precision,kernel", [(hexa.PRECOND_JACOBI, hexa.PREC_FP64, hexa.MATVEC_AUTO),
+                                                      (hexa.PRECOND_MG, hexa.PREC_FP64, hexa.MATVEC_AUTO),
+                                                      (hexa.PRECOND_MG, hexa.PREC_MIXED, hexa.MATVEC_AUTO),
+                                                      (hexa.PRECOND_MG, hexa.PREC_MIXED, hexa.MATVEC_DENSE)])
+def test_solve_matches_golden_direct(tag, precond, precision, kernel):
+    say("golden solve", tag, precond, precision, kernel)
     grid, fixed, f, rho, ug = _golden_problem(tag)
-    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, matvec_kernel=kernel) as s:
         s.set_problem(fixed, f)
         s.set_density(rho)
         u, reps, st = s.solve(tol=1e-12, precond=precond, semantics=hexa.SEMANTICS_CPU, shift_factor=1e-10,

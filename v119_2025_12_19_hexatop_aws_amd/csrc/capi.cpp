@@ -16,6 +16,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "handle.h"
 #include "hexa_internal.h"
 #include "ke.h"
 
@@ -32,83 +33,13 @@ void set_error(const char* fmt, ...) {
     g_err = buf;
 }
 
-namespace {
-
-constexpr int RING = 4;       // pinned state slots for the look-ahead poll
-constexpr int LOOKAHEAD = 2;  // iterations queued beyond the last one inspected
-
-double now_ms() {
-    return std::chrono::duration<double, std::milli>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
-struct Level {
-    Grid3 g{};
-    double kscale = 1.0;            // Ke(level) = kscale * Ke(fine)
-    double shift = 0.0;
-    double* Ke64 = nullptr;
-    float* Ke32 = nullptr;
-    double* fac64 = nullptr;        // owned for l >= 1
-    float* fac32 = nullptr;
-    uint8_t* mask = nullptr;        // owned for l >= 1
-    double* diag = nullptr;
-    void* dinv = nullptr;
-    void* b = nullptr;              // rhs (owned for l >= 1, or l == 0 in mixed mode)
-    void* x = nullptr;
-    void* xt = nullptr;
-    void* r = nullptr;
-};
-
-}  // namespace
 }  // namespace hexa
 
 using namespace hexa;
 
-struct hexa_handle {
-    hexa_grid grid{};
-    hexa_options opt{};
-    Grid3 g{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    hipEvent_t ring_ev[RING] = {};
-    double last_kernel_ms = 0.0;
+namespace hexa {
 
-    double Ke[576];
-    double* d_Ke64 = nullptr;
-    float* d_Ke32 = nullptr;
-
-    uint8_t* d_fixed = nullptr;    // 3*nn
-    uint8_t* d_active = nullptr;   // ne or null
-    uint8_t* d_mask = nullptr;     // nn
-    double* d_f = nullptr;         // nrhs * 3nn
-    double* d_rho = nullptr;
-    double* d_fac = nullptr;
-    float* d_fac32 = nullptr;
-    bool have_problem = false, have_density = false, fac32_valid = false;
-
-    double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_z = nullptr;
-    double *d_minv = nullptr, *d_best = nullptr, *d_b = nullptr, *d_diag = nullptr, *d_guess = nullptr;
-    double* d_partials = nullptr;
-    double* d_scal = nullptr;
-    PcgState* d_state = nullptr;
-    PcgState* h_state = nullptr;   // pinned ring
-
-    // preconditioner set-up of the last solve
-    int precond_ready = -1;        // HEXA_PRECOND_* or -1
-    std::vector<Level> levels;
-    double* d_Ainv = nullptr;
-    double *d_gj_row = nullptr, *d_gj_col = nullptr;
-    int64_t n_coarse = 0;
-    double shift = 0.0;
-
-    std::vector<void*> allocs;
-};
-
-namespace {
-
-int32_t dalloc(hexa_handle* h, void** p, size_t bytes) {
+int32_t dalloc_bytes(hexa_handle* h, void** p, size_t bytes) {
     if (*p) return HEXA_OK;
     hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
     if (e != hipSuccess) {
@@ -119,10 +50,6 @@ int32_t dalloc(hexa_handle* h, void** p, size_t bytes) {
     h->allocs.push_back(*p);
     return HEXA_OK;
 }
-template <typename T>
-int32_t dalloc(hexa_handle* h, T** p, size_t count) {
-    return dalloc(h, reinterpret_cast<void**>(p), count * sizeof(T));
-}
 
 int32_t check_launch() {
     hipError_t e = hipGetLastError();
@@ -131,6 +58,39 @@ int32_t check_launch() {
         return HEXA_E_DEVICE;
     }
     return HEXA_OK;
+}
+
+int32_t need_ready(hexa_handle* h) {
+    if (!h) { set_error("null handle"); return HEXA_E_INVALID; }
+    if (!h->have_problem) { set_error("hexa_set_problem has not been called"); return HEXA_E_INVALID; }
+    if (!h->have_density) { set_error("hexa_set_density has not been called"); return HEXA_E_INVALID; }
+    return HEXA_OK;
+}
+
+int32_t poll_state(hexa_handle* h, int k, bool* halted) {
+    const int slot = k % RING;
+    HX_CHECK(hipMemcpyAsync(&h->h_state[slot], h->d_state, sizeof(PcgState), hipMemcpyDeviceToHost, h->stream));
+    HX_CHECK(hipEventRecord(h->ring_ev[slot], h->stream));
+    *halted = false;
+    if (k - LOOKAHEAD >= 1) {
+        const int kk = k - LOOKAHEAD;
+        HX_CHECK(hipEventSynchronize(h->ring_ev[kk % RING]));
+        *halted = h->h_state[kk % RING].halted != PCG_RUNNING;
+    }
+    return HEXA_OK;
+}
+
+}  // namespace hexa
+
+namespace {
+
+// Elasticity operator on one grid: factored kernel for the exact brick
+// stiffness, dense-Ke kernel otherwise (e.g. the reference's f32-widened Ke).
+template <typename T, typename DT>
+void elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
+             const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y) {
+    if (h->factored) launch_elastic_fact<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
+    else launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
 }
 
 int32_t ensure_fac32(hexa_handle* h) {
@@ -159,7 +119,7 @@ int choose_levels(const hexa_handle* h) {
         return L;
     }
     while (nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 &&
-           3LL * (nx + 1) * (ny + 1) * (nz + 1) > 2000) {
+           3LL * (nx + 1) * (ny + 1) * (nz + 1) > 1200) {
         nx /= 2; ny /= 2; nz /= 2; ++L;
     }
     return L;
@@ -182,6 +142,8 @@ int32_t mg_build(hexa_handle* h) {
         for (int i = 0; i < 576; ++i) ke[i] = lv.kscale * h->Ke[i];
         HX_TRY(dalloc(h, &lv.Ke64, 576));
         HX_CHECK(hipMemcpyAsync(lv.Ke64, ke, sizeof ke, hipMemcpyHostToDevice, h->stream));
+        lv.c64 = make_fact_coef<double>(lv.kscale * h->grid.dx, lv.kscale * h->grid.dy, lv.kscale * h->grid.dz, h->opt.nu);
+        lv.c32 = make_fact_coef<float>(lv.kscale * h->grid.dx, lv.kscale * h->grid.dy, lv.kscale * h->grid.dz, h->opt.nu);
         if (mx) {
             float k32[576];
             for (int i = 0; i < 576; ++i) k32[i] = (float)ke[i];
@@ -237,6 +199,7 @@ int32_t mg_build(hexa_handle* h) {
 // shift s of the fine operator is carried to level l as 8^l s (row sum of
 // P^T P for the trilinear transfer).
 int32_t mg_set_shift(hexa_handle* h, double s) {
+    if (h->mg_shift == s) return HEXA_OK;
     const bool mx = mixed(h);
     const int L = (int)h->levels.size();
     for (int l = 0; l < L; ++l) {
@@ -254,6 +217,7 @@ int32_t mg_set_shift(hexa_handle* h, double s) {
     launch_coarse_assemble(h->stream, c.g, c.Ke64, c.fac64, c.mask, c.shift, h->d_Ainv);
     launch_gauss_jordan(h->stream, h->n_coarse, h->d_Ainv, h->d_gj_row, h->d_gj_col);
     HX_TRY(check_launch());
+    h->mg_shift = s;
     return HEXA_OK;
 }
 
@@ -269,6 +233,8 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out) {
     }
     const T* Ke = std::is_same<T, float>::value ? (const T*)lv.Ke32 : (const T*)lv.Ke64;
     const T* fac = std::is_same<T, float>::value ? (const T*)lv.fac32 : (const T*)lv.fac64;
+    const FactCoef<T>& coef = *reinterpret_cast<const FactCoef<T>*>(
+        std::is_same<T, float>::value ? (const void*)&lv.c32 : (const void*)&lv.c64);
     const DT* dinv = (const DT*)lv.dinv;
     const T omega = (T)h->opt.mg_omega;
     const T shift = (T)lv.shift;
@@ -278,16 +244,16 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out) {
     T* xb = (swaps % 2 == 0) ? (T*)lv.xt : out;
     launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa);
     for (int it = 1; it < pre; ++it) {
-        launch_elastic_op<T, DT>(s, lv.g, OP_SMOOTH, Ke, fac, lv.mask, xa, b, dinv, omega, shift, xb);
+        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb);
         std::swap(xa, xb);
     }
-    launch_elastic_op<T, DT>(s, lv.g, OP_RESID, Ke, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r);
+    elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r);
     Level& nx = h->levels[l + 1];
     launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b);
     vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x);
     launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa);
     for (int it = 0; it < post; ++it) {
-        launch_elastic_op<T, DT>(s, lv.g, OP_SMOOTH, Ke, fac, lv.mask, xa, b, dinv, omega, shift, xb);
+        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb);
         std::swap(xa, xb);
     }
 }
@@ -309,22 +275,6 @@ void apply_mg(hexa_handle* h, const double* r, double* z) {
     }
 }
 
-int32_t ensure_solver_vectors(hexa_handle* h) {
-    const size_t nd = (size_t)(3 * h->g.nn);
-    HX_TRY(dalloc(h, &h->d_x, nd));
-    HX_TRY(dalloc(h, &h->d_r, nd));
-    HX_TRY(dalloc(h, &h->d_p, nd));
-    HX_TRY(dalloc(h, &h->d_Ap, nd));
-    HX_TRY(dalloc(h, &h->d_z, nd));
-    HX_TRY(dalloc(h, &h->d_minv, nd));
-    HX_TRY(dalloc(h, &h->d_best, nd));
-    HX_TRY(dalloc(h, &h->d_b, nd));
-    HX_TRY(dalloc(h, &h->d_diag, nd));
-    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
-    HX_TRY(dalloc(h, &h->d_scal, 16));
-    HX_TRY(dalloc(h, &h->d_state, 1));
-    return HEXA_OK;
-}
 
 // b = f masked to the free DOFs.
 __global__ void k_mask_copy(int64_t nd, const double* f, const uint8_t* __restrict__ mask, double* b) {
@@ -361,8 +311,8 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
     // state, r = b - A x, z = M^-1 r, p = z, rz
     launch_dot2(st, nd, h->d_b, h->d_b, h->d_partials);
     launch_pcg_init_state(st, h->d_state, h->d_partials, prm->tol);
-    launch_elastic_op<double, double>(st, h->g, OP_RESID, h->d_Ke64, h->d_fac, h->d_mask, h->d_x, h->d_b,
-                                      nullptr, 0.0, s, h->d_r);
+    elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_RESID, h->d_fac, h->d_mask, h->d_x, h->d_b,
+                            nullptr, 0.0, s, h->d_r);
     ++*matvecs;
     if (use_mg) apply_mg(h, h->d_r, h->d_z);
     else launch_scale_dinv<double, double>(st, nd, h->d_r, h->d_minv, 1.0, h->d_z);
@@ -373,10 +323,9 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
 
     int k_done = 0;
     bool stop = false;
-    int inspected = 0;
     for (int k = 1; k <= prm->max_iter && !stop; ++k) {
-        launch_elastic_op<double, double>(st, h->g, OP_APPLY, h->d_Ke64, h->d_fac, h->d_mask, h->d_p,
-                                          nullptr, nullptr, 0.0, s, h->d_Ap);
+        elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask, h->d_p,
+                                nullptr, nullptr, 0.0, s, h->d_Ap);
         ++*matvecs;
         launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
         launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
@@ -394,18 +343,9 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, nullptr, h->d_r, h->d_minv);
         }
         HX_TRY(check_launch());
-        const int slot = k % RING;
-        HX_CHECK(hipMemcpyAsync(&h->h_state[slot], h->d_state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
-        HX_CHECK(hipEventRecord(h->ring_ev[slot], st));
         k_done = k;
-        if (k - LOOKAHEAD >= 1) {
-            const int kk = k - LOOKAHEAD;
-            HX_CHECK(hipEventSynchronize(h->ring_ev[kk % RING]));
-            inspected = kk;
-            if (h->h_state[kk % RING].halted != PCG_RUNNING) stop = true;
-        }
+        HX_TRY(poll_state(h, k, &stop));
     }
-    (void)inspected;
     HX_CHECK(hipMemcpyAsync(&h->h_state[0], h->d_state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
     HX_CHECK(hipStreamSynchronize(st));
     const PcgState& fs = h->h_state[0];
@@ -416,12 +356,6 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
     return HEXA_OK;
 }
 
-int32_t need_ready(hexa_handle* h) {
-    if (!h) { set_error("null handle"); return HEXA_E_INVALID; }
-    if (!h->have_problem) { set_error("hexa_set_problem has not been called"); return HEXA_E_INVALID; }
-    if (!h->have_density) { set_error("hexa_set_density has not been called"); return HEXA_E_INVALID; }
-    return HEXA_OK;
-}
 
 int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_params* prm,
                   hexa_solve_report* rep) {
@@ -461,11 +395,20 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
             use_mg = false;  // MG set-up failure falls back to Jacobi (GPUSolver.jl:259-267)
         } else {
             h->precond_ready = HEXA_PRECOND_MG;
+            h->mg_shift = -1.0;
         }
     }
-    if (prm->warm_start && u_inout) {
+    // the first attempt's smoother and coarse inverse belong to the set-up
+    if (use_mg) HX_TRY(mg_set_shift(h, prm->shift_factor * maxd));
+    // warm start: from the host buffer, or (u_inout == NULL) from the
+    // device-resident solution of the previous solve on this handle
+    const bool warm = prm->warm_start && (u_inout || h->have_solution);
+    if (warm) {
         HX_TRY(dalloc(h, &h->d_guess, (size_t)nd));
-        HX_CHECK(hipMemcpyAsync(h->d_guess, u_inout, nd * sizeof(double), hipMemcpyHostToDevice, st));
+        if (u_inout)
+            HX_CHECK(hipMemcpyAsync(h->d_guess, u_inout, nd * sizeof(double), hipMemcpyHostToDevice, st));
+        else
+            HX_CHECK(hipMemcpyAsync(h->d_guess, h->d_x, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
         k_mask_copy<<<grid_for(nd), 256, 0, st>>>(nd, h->d_guess, h->d_mask, h->d_guess);
     }
     launch_fill<double>(st, nd, 0.0, h->d_x);
@@ -499,7 +442,7 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
                 const double s = cur * maxd;
                 attempt_used = attempt;
                 shift_used = s;
-                if (prm->warm_start && u_inout && attempt == 1 && use_mg == want_mg) {
+                if (warm && attempt == 1 && use_mg == want_mg) {
                     HX_CHECK(hipMemcpyAsync(h->d_x, h->d_guess, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
                 } else if (attempt > 1 || use_mg != want_mg) {
                     launch_fill<double>(st, nd, 0.0, h->d_x);
@@ -533,8 +476,8 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     // compliance f.u and the true relative residual of the shifted system
     launch_dot2(st, nd, h->d_b, h->d_x, h->d_partials);
     launch_final_sum(st, 1, h->d_partials, h->d_scal);
-    launch_elastic_op<double, double>(st, h->g, OP_RESID, h->d_Ke64, h->d_fac, h->d_mask, h->d_x, h->d_b,
-                                      nullptr, 0.0, shift_used, h->d_r);
+    elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_RESID, h->d_fac, h->d_mask, h->d_x, h->d_b,
+                            nullptr, 0.0, shift_used, h->d_r);
     launch_dot2(st, nd, h->d_r, h->d_r, h->d_partials);
     launch_final_sum(st, 1, h->d_partials, h->d_scal + 1);
     launch_dot2(st, nd, h->d_b, h->d_b, h->d_partials);
@@ -544,6 +487,7 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     HX_CHECK(hipMemcpyAsync(sc, h->d_scal, sizeof sc, hipMemcpyDeviceToHost, st));
     if (u_inout) HX_CHECK(hipMemcpyAsync(u_inout, h->d_x, nd * sizeof(double), hipMemcpyDeviceToHost, st));
     HX_CHECK(hipStreamSynchronize(st));
+    h->have_solution = true;
     R.compliance = sc[0];
     R.true_rel_res = sc[2] > 0 ? std::sqrt(sc[1] / sc[2]) : 0.0;
     R.rel_res = final_rel;
@@ -560,6 +504,39 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
 }  // namespace
 
 // =================================================================== C ABI
+namespace hexa {
+int32_t ensure_solver_vectors(hexa_handle* h) {
+    const size_t nd = (size_t)(3 * h->g.nn);
+    HX_TRY(dalloc(h, &h->d_x, nd));
+    HX_TRY(dalloc(h, &h->d_r, nd));
+    HX_TRY(dalloc(h, &h->d_p, nd));
+    HX_TRY(dalloc(h, &h->d_Ap, nd));
+    HX_TRY(dalloc(h, &h->d_z, nd));
+    HX_TRY(dalloc(h, &h->d_minv, nd));
+    HX_TRY(dalloc(h, &h->d_best, nd));
+    HX_TRY(dalloc(h, &h->d_b, nd));
+    HX_TRY(dalloc(h, &h->d_diag, nd));
+    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
+    HX_TRY(dalloc(h, &h->d_scal, 16));
+    HX_TRY(dalloc(h, &h->d_state, 1));
+    return HEXA_OK;
+}
+
+int32_t update_factor(hexa_handle* h) {
+    hipStream_t st = h->stream;
+    HX_TRY(dalloc(h, &h->d_fac, (size_t)h->g.ne));
+    const uint8_t* act = (h->have_problem && h->d_active) ? h->d_active : nullptr;
+    launch_stiffness_factor<double>(st, h->g.ne, h->d_rho, act, h->opt.E0, h->opt.Emin, h->opt.penal,
+                                    h->d_fac);
+    HX_TRY(check_launch());
+    HX_CHECK(hipStreamSynchronize(st));
+    h->have_density = true;
+    h->fac32_valid = false;
+    h->precond_ready = -1;
+    return HEXA_OK;
+}
+}  // namespace hexa
+
 extern "C" {
 
 const char* hexa_last_error(void) { return g_err.c_str(); }
@@ -650,6 +627,15 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
         set_error("invalid element (nu must be in (-1, 0.5))");
         return HEXA_E_INVALID;
     }
+    if (o.matvec_kernel == HEXA_MATVEC_FACTORED && o.ke_mode != HEXA_KE_FP64) {
+        delete h;
+        set_error("the factored element kernel needs ke_mode = HEXA_KE_FP64 (exact brick stiffness)");
+        return HEXA_E_INVALID;
+    }
+    h->factored = o.matvec_kernel == HEXA_MATVEC_FACTORED ||
+                  (o.matvec_kernel == HEXA_MATVEC_AUTO && o.ke_mode == HEXA_KE_FP64);
+    h->coef64 = make_fact_coef<double>(grid->dx, grid->dy, grid->dz, o.nu);
+    h->coef32 = make_fact_coef<float>(grid->dx, grid->dy, grid->dz, o.nu);
     auto fail = [&](int32_t s) { hexa_destroy(h); return s; };
     if (hipGetDevice(&h->device) != hipSuccess) return fail(HEXA_E_DEVICE);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(HEXA_E_DEVICE);
@@ -701,35 +687,23 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
     h->have_problem = true;
     h->precond_ready = -1;
     h->have_density = false;  // the factor depends on the active set
+    h->have_solution = false;
     return HEXA_OK;
 }
 
-static int32_t finish_density(hexa_handle* h) {
-    hipStream_t st = h->stream;
-    HX_TRY(dalloc(h, &h->d_fac, (size_t)h->g.ne));
-    const uint8_t* act = (h->have_problem && h->d_active) ? h->d_active : nullptr;
-    launch_stiffness_factor<double>(st, h->g.ne, h->d_rho, act, h->opt.E0, h->opt.Emin, h->opt.penal,
-                                    h->d_fac);
-    HX_TRY(check_launch());
-    HX_CHECK(hipStreamSynchronize(st));
-    h->have_density = true;
-    h->fac32_valid = false;
-    h->precond_ready = -1;
-    return HEXA_OK;
-}
 
 int32_t hexa_set_density(hexa_handle* h, const double* rho) {
     if (!h || !rho) { set_error("null argument"); return HEXA_E_INVALID; }
     HX_TRY(dalloc(h, &h->d_rho, (size_t)h->g.ne));
     HX_CHECK(hipMemcpyAsync(h->d_rho, rho, sizeof(double) * h->g.ne, hipMemcpyHostToDevice, h->stream));
-    return finish_density(h);
+    return update_factor(h);
 }
 
 int32_t hexa_set_density_dev(hexa_handle* h, const double* d_rho) {
     if (!h || !d_rho) { set_error("null argument"); return HEXA_E_INVALID; }
     HX_TRY(dalloc(h, &h->d_rho, (size_t)h->g.ne));
     HX_CHECK(hipMemcpyAsync(h->d_rho, d_rho, sizeof(double) * h->g.ne, hipMemcpyDeviceToDevice, h->stream));
-    return finish_density(h);
+    return update_factor(h);
 }
 
 int32_t hexa_solve(hexa_handle* h, double* u_inout, const hexa_solve_params* prm, hexa_solve_report* rep) {
@@ -772,8 +746,8 @@ int32_t hexa_matvec(hexa_handle* h, const double* x, double* y) {
     HX_TRY(dalloc(h, &h->d_Ap, (size_t)nd));
     hipStream_t st = h->stream;
     HX_CHECK(hipMemcpyAsync(h->d_p, x, sizeof(double) * nd, hipMemcpyHostToDevice, st));
-    launch_elastic_op<double, double>(st, h->g, OP_APPLY, h->d_Ke64, h->d_fac, nullptr, h->d_p, nullptr,
-                                      nullptr, 0.0, 0.0, h->d_Ap);
+    elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, nullptr, h->d_p, nullptr,
+                            nullptr, 0.0, 0.0, h->d_Ap);
     HX_TRY(check_launch());
     HX_CHECK(hipMemcpyAsync(y, h->d_Ap, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
     HX_CHECK(hipStreamSynchronize(st));
@@ -788,11 +762,11 @@ int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t prec
     HX_CHECK(hipEventRecord(h->ev_a, st));
     for (int r = 0; r < reps; ++r) {
         if (precision == HEXA_PREC_MIXED)
-            launch_elastic_op<float, float>(st, h->g, OP_APPLY, h->d_Ke32, h->d_fac32, nullptr,
-                                            (const float*)d_x, nullptr, nullptr, 0.0f, 0.0f, (float*)d_y);
+            elastic<float, float>(h, h->g, h->coef32, h->d_Ke32, OP_APPLY, h->d_fac32, nullptr,
+                                  (const float*)d_x, nullptr, nullptr, 0.0f, 0.0f, (float*)d_y);
         else
-            launch_elastic_op<double, double>(st, h->g, OP_APPLY, h->d_Ke64, h->d_fac, nullptr,
-                                              (const double*)d_x, nullptr, nullptr, 0.0, 0.0, (double*)d_y);
+            elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, nullptr,
+                                    (const double*)d_x, nullptr, nullptr, 0.0, 0.0, (double*)d_y);
     }
     HX_CHECK(hipEventRecord(h->ev_b, st));
     HX_TRY(check_launch());

@@ -1,0 +1,117 @@
+// The libhexa handle and host helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <chrono>
+#include <vector>
+
+#include "hexa_internal.h"
+
+namespace hexa {
+
+constexpr int RING = 4;       // pinned state slots for the look-ahead poll
+constexpr int LOOKAHEAD = 2;  // iterations queued beyond the last one inspected
+
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Level {
+    Grid3 g{};
+    double kscale = 1.0;            // Ke(level) = kscale * Ke(fine)
+    double shift = 0.0;
+    double* Ke64 = nullptr;
+    float* Ke32 = nullptr;
+    FactCoef<double> c64{};
+    FactCoef<float> c32{};
+    double* fac64 = nullptr;        // owned for l >= 1
+    float* fac32 = nullptr;
+    uint8_t* mask = nullptr;        // owned for l >= 1
+    double* diag = nullptr;
+    void* dinv = nullptr;
+    void* b = nullptr;              // rhs (owned for l >= 1, or l == 0 in mixed mode)
+    void* x = nullptr;
+    void* xt = nullptr;
+    void* r = nullptr;
+};
+
+}  // namespace hexa
+
+struct hexa_handle {
+    hexa_grid grid{};
+    hexa_options opt{};
+    hexa::Grid3 g{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipEvent_t ring_ev[hexa::RING] = {};
+    double last_kernel_ms = 0.0;
+
+    double Ke[576];
+    bool factored = false;          // exact brick Ke -> factored element kernel
+    hexa::FactCoef<double> coef64{};
+    hexa::FactCoef<float> coef32{};
+    double* d_Ke64 = nullptr;
+    float* d_Ke32 = nullptr;
+
+    uint8_t* d_fixed = nullptr;    // 3*nn
+    uint8_t* d_active = nullptr;   // ne or null
+    uint8_t* d_mask = nullptr;     // nn
+    double* d_f = nullptr;         // nrhs * 3nn
+    double* d_rho = nullptr;
+    double* d_fac = nullptr;
+    float* d_fac32 = nullptr;
+    bool have_problem = false, have_density = false, fac32_valid = false, have_solution = false;
+
+    double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_z = nullptr;
+    double *d_minv = nullptr, *d_best = nullptr, *d_b = nullptr, *d_diag = nullptr, *d_guess = nullptr;
+    double* d_partials = nullptr;
+    double* d_scal = nullptr;
+    hexa::PcgState* d_state = nullptr;
+    hexa::PcgState* h_state = nullptr;   // pinned ring
+
+    // preconditioner set-up of the last solve
+    int precond_ready = -1;        // HEXA_PRECOND_* or -1
+    std::vector<hexa::Level> levels;
+    double* d_Ainv = nullptr;
+    double *d_gj_row = nullptr, *d_gj_col = nullptr;
+    int64_t n_coarse = 0;
+    double mg_shift = -1.0;         // shift the smoother/coarse inverse were built for
+
+    // Helmholtz filter and optimisation-step work arrays
+    double* d_Ae = nullptr;         // 64
+    double *d_hb = nullptr, *d_hx = nullptr, *d_hr = nullptr, *d_hp = nullptr, *d_hAp = nullptr, *d_hdinv = nullptr;
+    double *d_e0 = nullptr, *d_e1 = nullptr, *d_e2 = nullptr, *d_e3 = nullptr, *d_e4 = nullptr, *d_e5 = nullptr;
+    double *d_e6 = nullptr, *d_ones = nullptr;
+    float* d_stress = nullptr;      // 17 nE floats of stress outputs
+    double w_radius = -1.0;         // radius the filter weights d_w were computed for
+    double* d_w = nullptr;
+    float *d_f0 = nullptr, *d_f1 = nullptr, *d_f2 = nullptr;
+    uint8_t *d_u0 = nullptr, *d_u1 = nullptr;
+    void* d_bracket = nullptr;
+    unsigned long long* d_hist = nullptr;
+    int64_t* d_ties = nullptr;
+
+    std::vector<void*> allocs;
+};
+
+namespace hexa {
+
+int32_t dalloc_bytes(hexa_handle* h, void** p, size_t bytes);
+inline int32_t dalloc(hexa_handle* h, void** p, size_t bytes) { return dalloc_bytes(h, p, bytes); }
+template <typename T>
+inline int32_t dalloc(hexa_handle* h, T** p, size_t count) {
+    return dalloc_bytes(h, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+int32_t check_launch();
+int32_t need_ready(hexa_handle* h);
+// Recomputes the stiffness multiplier from d_rho (after the density changed).
+int32_t update_factor(hexa_handle* h);
+int32_t ensure_solver_vectors(hexa_handle* h);
+// Waits for the device state of iteration k - LOOKAHEAD (ring slot) and
+// reports whether it halted; records iteration k's state copy.
+int32_t poll_state(hexa_handle* h, int k, bool* halted);
+// Helmholtz filter of an element field on the handle's grid (device arrays).
+int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double radius, double tol, int max_iter,
+                   int check_every, int* iters, double* rel);
+
+}  // namespace hexa

@@ -35,7 +35,8 @@ void set_error(const char* fmt, ...);
 // independent of what else shares the GPU.
 constexpr int RED_BLOCKS = 1024;
 constexpr int RED_THREADS = 256;
-constexpr int MAX_RED_Q = 4;
+constexpr int OC_NL = 32;      // lambda points per multisection pass of the OC update
+constexpr int MAX_RED_Q = 32;  // partial-sum rows per reduction buffer
 
 // Node mask bits.  Bits 0..2: component held.  Bit 3: node excluded (no
 // active element touches it; all three components are then held).
@@ -59,16 +60,16 @@ inline Grid3 make_grid(int nx, int ny, int nz) {
     return g;
 }
 
-// Factored element operator (exact brick stiffness).  In the 2x2x2
-// Hadamard ("modal") basis of each displacement component the hex8 brick
-// stiffness is almost diagonal: per component i a diagonal over the 8
-// modes, plus, for each ordered component pair (i,j), two couplings of two
-// modes each.  All coefficients are scaled by the level's size.
-struct FactoredOp {
-    double diag[3][8];   // modal diagonal per output component
-    double lam[3];       // lambda * Mz-type mass for the (x,y),(y,z),(x,z) couplings, per mode parity
-    double mu[3];
-    double cpl[3][2][2]; // [pair][which][m] coupling coefficient
+// Factored element operator (exact brick stiffness).  In the per-axis
+// sum/difference ("modal") basis of each displacement component the hex8
+// brick stiffness is a diagonal per component and mode plus, for each
+// ordered component pair (i,j) with third axis d, two couplings per mode
+// parity q of d.  Coefficients carry the 1/8 of the inverse butterfly.
+template <typename T>
+struct FactCoef {
+    T D[3][8];    // modal diagonal, per output component, mode m = mx + 2my + 4mz
+    T cl[3][2];   // lambda-term coupling, per third axis d and its mode q
+    T cm[3][2];   // mu-term coupling
 };
 
 // Device-side state of one PCG recurrence.  The iteration's scalars live on
@@ -98,6 +99,11 @@ template <typename T, typename DT>
 void launch_elastic_op(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, const T* fac,
                        const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega,
                        T shift, T* y);
+template <typename T, typename DT>
+void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
+                         const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
+template <typename T>
+FactCoef<T> make_fact_coef(double hx, double hy, double hz, double nu);
 template <typename T>
 void launch_stiffness_factor(hipStream_t s, int64_t ne, const double* rho, const uint8_t* active,
                              double E0, double Emin, double penal, T* fac);
@@ -140,6 +146,37 @@ void launch_max_free(hipStream_t s, const Grid3& g, const double* diag, const ui
 void launch_final_sum(hipStream_t s, int nq, const double* partials, double* out);
 template <typename T>
 void launch_axpy(hipStream_t s, int64_t n, T a, const T* x, T* y);
+
+// Helmholtz filter (k_filter.hip)
+void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const double* x, double* y);
+void launch_helm_rhs_diag(hipStream_t s, const Grid3& g, const double* Ae, const double* rho, double w, double* b,
+                          double* dinv);
+void launch_helm_extract(hipStream_t s, const Grid3& g, const double* x, double scale, double* out);
+
+// Optimisation step (k_topopt.hip)
+void launch_stress(hipStream_t s, const Grid3& g, double dx, double dy, double dz, double E, double nu,
+                   const double* U, const double* rho, float* principal, float* vm, float* l1, float* dmax,
+                   float* dmin, float* voigt);
+void launch_simp_sens(hipStream_t s, const Grid3& g, const double* Ke, const double* rho, double E0, double Emin,
+                      double penal, const double* U, double* dc, double* partials);
+int oc_bracket_bytes();
+void launch_oc(hipStream_t s, int64_t n, const double* dc, const double* dv, const double* w, double volfrac,
+               double move, double tol, int max_passes, double* x, double* q, double* lo, double* hi, void* bracket,
+               double* partials, double* out4);
+void launch_upd_proposed(hipStream_t s, int64_t n, const float* l1, const uint8_t* prot, const float* orig, float allow,
+                         float E, float mind, double* prop);
+void launch_nan_count(hipStream_t s, int64_t n, const double* x, double* partials);
+void launch_upd_apply(hipStream_t s, int64_t n, const double* filt, const double* prop, int use_prop,
+                      const uint8_t* prot, float mind, float maxd, float* dens, double* partials);
+void launch_upd_candidates(hipStream_t s, int64_t n, const uint8_t* prot, const float* orig, float mind, float maxd,
+                           float thr, float* dens, uint8_t* cand, double* partials);
+void launch_radix_hist(hipStream_t s, int64_t n, const uint8_t* cand, const float* dens, uint32_t prefix, int shift,
+                       unsigned long long* hist);
+void launch_tie_count(hipStream_t s, int64_t n, const uint8_t* cand, const float* dens, uint32_t key, int64_t* counts);
+void launch_cull(hipStream_t s, int64_t n, const uint8_t* cand, uint32_t key_t, int64_t ties, const int64_t* tie_base,
+                 float mind, float* dens);
+void launch_upd_finish(hipStream_t s, int64_t n, const uint8_t* prot, const float* orig, float mind, int hard,
+                       float* dens);
 
 // PCG (k_pcg.hip).  All take the device state and do nothing once it halted.
 void launch_pcg_init_state(hipStream_t s, PcgState* st, const double* partials, double tol);

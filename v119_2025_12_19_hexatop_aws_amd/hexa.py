@@ -76,6 +76,42 @@ class SolveReport(C.Structure):
         return d
 
 
+class FilterReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("pad", C.c_int32), ("rel_res", C.c_double), ("ms", C.c_double)]
+
+
+class OcParams(C.Structure):
+    _fields_ = [("volfrac", C.c_double), ("move", C.c_double), ("rmin", C.c_double), ("filter_tol", C.c_double),
+                ("bisect_tol", C.c_double), ("filter_max_iter", C.c_int32), ("pad", C.c_int32),
+                ("solve", SolveParams)]
+
+
+class OcReport(C.Structure):
+    _fields_ = [("compliance", C.c_double), ("volume", C.c_double), ("change", C.c_double), ("lambda_", C.c_double),
+                ("solve_ms", C.c_double), ("step_ms", C.c_double), ("solve_iterations", C.c_int32),
+                ("filter_iterations", C.c_int32), ("status", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {k.rstrip("_"): getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class UpdateParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "E", "l1_allow", "min_density", "max_density", "max_domain_dim", "filter_tol", "R_init_perc",
+        "R_interm_perc", "R_final_perc", "R_interm_iter_perc", "min_radius_factor", "final_threshold",
+        "max_culling_ratio")] + [("iter", C.c_int32), ("n_iters", C.c_int32), ("hard", C.c_int32),
+                                 ("pad", C.c_int32)]
+
+
+class UpdateReport(C.Structure):
+    _fields_ = [("mean_change", C.c_double), ("R_length", C.c_double), ("threshold", C.c_double),
+                ("filter_res", C.c_double), ("filter_iters", C.c_int32), ("skipped", C.c_int32),
+                ("culled", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _H = C.c_void_p
 _P = C.c_void_p
 
@@ -99,6 +135,14 @@ SIGNATURES = {
     "hexa_apply_precond": (C.c_int32, [_H, _P, _P]),
     "hexa_last_kernel_ms": (C.c_int32, [_H, C.POINTER(C.c_double)]),
     "hexa_stream_sync": (C.c_int32, [_H]),
+    "hexa_filter": (C.c_int32, [_H, _P, _P, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                C.POINTER(FilterReport)]),
+    "hexa_stress": (C.c_int32, [_H, _P, _P, C.c_double, _P, _P, _P, _P, _P, _P]),
+    "hexa_sensitivity": (C.c_int32, [_H, _P, C.POINTER(C.c_double)]),
+    "hexa_oc_update": (C.c_int32, [_H, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(C.c_double)]),
+    "hexa_oc_step": (C.c_int32, [_H, _P, C.POINTER(OcParams), C.POINTER(OcReport)]),
+    "hexa_update_density": (C.c_int32, [_H, _P, _P, _P, _P, C.POINTER(UpdateParams), C.POINTER(UpdateReport)]),
 }
 
 _lib = None
@@ -239,6 +283,64 @@ class Solver:
         _check(lib().hexa_apply_precond(self._h, _ptr(r), _ptr(z)))
         return z
 
+    def filter(self, rho, radius, tol=1e-6, max_iter=200, check_every=10):
+        """Helmholtz filter of an element field (GPUHelmholtz.jl:190-266 semantics)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(rho)
+        rep = FilterReport()
+        _check(lib().hexa_filter(self._h, _ptr(rho), _ptr(out), float(radius), float(tol), int(max_iter),
+                                 int(check_every), C.byref(rep)))
+        return out, rep.iterations, rep.rel_res
+
+    def stress(self, u=None, rho=None, E=1.0):
+        ne = self.ne
+        out = dict(principal=np.zeros(3 * ne, np.float32), vonmises=np.zeros(ne, np.float32),
+                   l1=np.zeros(ne, np.float32), dir_max=np.zeros(3 * ne, np.float32),
+                   dir_min=np.zeros(3 * ne, np.float32), voigt=np.zeros(6 * ne, np.float32))
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        rr = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        _check(lib().hexa_stress(self._h, _ptr(uu), _ptr(rr), float(E), _ptr(out["principal"]),
+                                 _ptr(out["vonmises"]), _ptr(out["l1"]), _ptr(out["dir_max"]), _ptr(out["dir_min"]),
+                                 _ptr(out["voigt"])))
+        for k in ("principal", "dir_max", "dir_min"):
+            out[k] = out[k].reshape(ne, 3)
+        out["voigt"] = out["voigt"].reshape(ne, 6)
+        return out
+
+    def sensitivity(self):
+        dc = np.zeros(self.ne)
+        c = C.c_double()
+        _check(lib().hexa_sensitivity(self._h, _ptr(dc), C.byref(c)))
+        return c.value, dc
+
+    def oc_update(self, x, dc, dv, w, volfrac, move, bisect_tol=1e-3):
+        x = np.array(x, dtype=np.float64).reshape(-1)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        dc, dv, w = f(dc), f(dv), f(w)
+        lam = C.c_double()
+        _check(lib().hexa_oc_update(self._h, _ptr(x), _ptr(dc), _ptr(dv), _ptr(w), float(volfrac), float(move),
+                                    float(bisect_tol), C.byref(lam)))
+        return x, lam.value
+
+    def oc_step(self, x, volfrac, move, rmin, filter_tol=1e-10, filter_max_iter=500, bisect_tol=1e-12, **solve):
+        x = np.array(x, dtype=np.float64).reshape(-1)
+        p = OcParams(volfrac=volfrac, move=move, rmin=rmin, filter_tol=filter_tol, bisect_tol=bisect_tol,
+                     filter_max_iter=filter_max_iter, solve=default_params(**solve))
+        rep = OcReport()
+        st = lib().hexa_oc_step(self._h, _ptr(x), C.byref(p), C.byref(rep))
+        _check(st, allow=(OK, E_NOT_CONVERGED))
+        return x, rep.as_dict()
+
+    def update_density(self, density, l1, protected, original, **params):
+        d = np.array(density, dtype=np.float32).reshape(-1)
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+        p = UpdateParams(**params)
+        rep = UpdateReport()
+        _check(lib().hexa_update_density(self._h, _ptr(d), _ptr(f32(l1)),
+                                         _ptr(np.ascontiguousarray(protected, dtype=np.uint8).reshape(-1)),
+                                         _ptr(f32(original)), C.byref(p), C.byref(rep)))
+        return d, rep.as_dict()
+
     def displacement(self):
         u = np.zeros(self.ndof)
         _check(lib().hexa_get_displacement(self._h, _ptr(u)))
@@ -364,3 +466,86 @@ def solve_system(nodes, elements, E, nu, bc, F, *, density=(), config, min_stiff
     full = np.zeros((len(nodes), 3))
     full[new_to_old] = np.asarray(U).reshape(-1, 3)
     return full.reshape(-1), res, meth
+
+
+def _grid_from_nodes(nodes, elements):
+    """Structured box and element size from generate_mesh-style arrays
+    (element size from element 1's edges, as CPUSolver.jl:41-42)."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    el = np.asarray(elements)
+    n1, n2, n4, n5 = (nodes[el[0, i] - 1] for i in (0, 1, 3, 4))
+    h = (float(np.linalg.norm(n2 - n1)), float(np.linalg.norm(n4 - n1)), float(np.linalg.norm(n5 - n1)))
+    ext = nodes.max(axis=0) - nodes.min(axis=0)
+    return tuple(int(round(ext[d] / h[d])) for d in range(3)), h
+
+
+_FILTER_CACHE = {}
+
+
+def apply_gpu_filter(density, nElem_x, nElem_y, nElem_z, dx, dy, dz, radius, elements=None, tol=1.0e-5,
+                     helm_element=HELM_BUG_COMPAT):
+    """Mirror of GPUHelmholtz.apply_gpu_filter! (GPUHelmholtz.jl:422-443):
+    returns (filtered Float32 vector, time_s, iterations, rel_res); an empty
+    vector signals failure (the caller's CPU fallback, TopOpt.jl:319-328)."""
+    try:
+        key = (int(nElem_x), int(nElem_y), int(nElem_z), float(dx), float(dy), float(dz), helm_element)
+        s = _FILTER_CACHE.get(key)
+        if s is None:
+            s = Solver(key[0], key[1], key[2], key[3:6], helm_element=helm_element)
+            _FILTER_CACHE.clear()
+            _FILTER_CACHE[key] = s
+        out, it, rel = s.filter(np.asarray(density, dtype=np.float64), radius, tol=float(np.float32(tol)),
+                                max_iter=200, check_every=10)
+        return out.astype(np.float32), 0.0, it, rel
+    except (HexaError, ImportError, OSError) as e:  # reference: any exception -> Float32[]
+        print(f"  [GPU Filter] CRITICAL ERROR: {e}. Falling back to CPU Filter...")
+        return np.zeros(0, np.float32), 0.0, 0, 0.0
+
+
+def compute_stress_field(nodes, elements, U, E, nu, density, return_voigt=True):
+    """Mirror of Stress.compute_stress_field (Stress.jl:287-374) on the
+    structured box: returns (principal 3 x nE, von Mises, voigt 6 x nE (or
+    0 x 0), l1, max direction 3 x nE, min direction 3 x nE), Float32."""
+    (nx, ny, nz), h = _grid_from_nodes(nodes, elements)
+    nid, eid = _lattice(nodes, elements, nx, ny, nz, h)
+    nN = (nx + 1) * (ny + 1) * (nz + 1)
+    u = np.zeros((nN, 3))
+    u[nid] = np.asarray(U, dtype=np.float64).reshape(-1, 3)
+    rho = np.zeros(nx * ny * nz)
+    rho[eid] = np.asarray(density, dtype=np.float64)
+    with Solver(nx, ny, nz, h, nu=float(nu)) as s:
+        o = s.stress(u.reshape(-1), rho, E=float(E))
+    pick = lambda a: np.ascontiguousarray(a[eid].T)
+    voigt = pick(o["voigt"]) if return_voigt else np.zeros((0, 0), np.float32)
+    return pick(o["principal"]), o["vonmises"][eid], voigt, o["l1"][eid], pick(o["dir_max"]), pick(o["dir_min"])
+
+
+def update_density(density, l1_stress_norm_field, protected_elements_mask, E, l1_stress_allowable, iter,
+                   number_of_iterations, original_density, min_density, max_density, config, elements=None,
+                   is_annealing=False, helm_element=HELM_BUG_COMPAT):
+    """Mirror of TopologyOptimization.update_density! (TopOpt.jl:211-482):
+    mutates ``density`` (Float32) and returns (mean_change, R_length,
+    threshold, filter_time, filter_iters, filter_res)."""
+    g = config["geometry"]
+    op = config["optimization_parameters"]
+    sp = config.get("solver_parameters", {})
+    nx, ny, nz = int(g["nElem_x_computed"]), int(g["nElem_y_computed"]), int(g["nElem_z_computed"])
+    h = (float(g["dx_computed"]), float(g["dy_computed"]), float(g["dz_computed"]))
+    prm = dict(E=float(E), l1_allow=float(l1_stress_allowable), min_density=float(min_density),
+               max_density=float(max_density), max_domain_dim=float(g["max_domain_dim"]),
+               filter_tol=float(np.float32(sp.get("filter_tolerance", 1.0e-5))),
+               R_init_perc=float(op.get("filter_R_init_perc", 0.0)),
+               R_interm_perc=float(op.get("filter_R_interm_perc", 0.0)),
+               R_final_perc=float(op.get("filter_R_final_perc", 0.0)),
+               R_interm_iter_perc=float(op.get("filter_R_interm_iter_perc", 50.0)),
+               min_radius_factor=float(op.get("minimum_filter_radius_as_elements", 2.5)),
+               final_threshold=float(op.get("final_density_threshold", 0.95)),
+               max_culling_ratio=float(op.get("max_culling_ratio", 0.1)), iter=int(iter),
+               n_iters=int(number_of_iterations), hard=int(op.get("density_update_method", "soft") == "hard"))
+    with Solver(nx, ny, nz, h, helm_element=helm_element) as s:
+        d, rep = s.update_density(density, l1_stress_norm_field, np.asarray(protected_elements_mask, dtype=np.uint8),
+                                  original_density, **prm)
+    if rep["skipped"]:
+        return 0.0, 0.0, 0.0, 0.0, 0, 0.0
+    density[:] = d
+    return rep["mean_change"], rep["R_length"], rep["threshold"], 0.0, rep["filter_iters"], rep["filter_res"]

@@ -161,7 +161,9 @@ def cantilever_c2() -> Problem:
 
 def cantilever_c3() -> Problem:
     p = cantilever(512, 256, 256, seed=3, smooth_passes=3)
-    p.meta.update(mg_levels=6)
+    # 6 levels leave a 16x8x8 coarsest grid (4131 DOFs); automatic level
+    # selection goes one level further (8x4x4, 675 DOFs)
+    p.meta.update(mg_levels_config=6)
     return p
 
 
