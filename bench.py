@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--jacobi-max-iter", type=int, default=30000)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="also run one plain device copy of 8*nDOF bytes (the reference kernel that scales the "
+                         "FETCH_SIZE/WRITE_SIZE counters of a rocprofv3 --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -117,6 +120,9 @@ def main():
     achieved = nbytes64 / (ms_kernel * 1e-3) / 1e9
     log(f"FP64 matvec: {ms_kernel:.3f} ms/launch (events), {ms_step:.3f} ms/step wall, {achieved:.0f} GB/s; "
         f"FP32 matvec {ms32:.3f} ms ({nbytes32 / ms32 / 1e6:.0f} GB/s)")
+    if args.traffic_probe:
+        y.copy_(x)
+        torch.cuda.synchronize()
     del x, y, x32, y32
     torch.cuda.empty_cache()
 
@@ -168,6 +174,12 @@ def main():
         cpu = cpu_baseline(p)
         log("cpu", cpu)
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"matvec_traffic_{args.config}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
     out = {
         "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -176,7 +188,7 @@ def main():
                                f"({p.n_elem} elements, {p.n_dof} DOFs), SIMP p=3, FP64",
                    "grid": [p.nx, p.ny, p.nz], "n_elem": p.n_elem, "n_dof": p.n_dof},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_elastic_fact<double>", "bytes_per_launch": nbytes64,
                      "kernel_ms": ms_kernel},
         "matvec_fp32": {"kernel_ms": ms32, "GBps": nbytes32 / (ms32 * 1e-3) / 1e9},

@@ -22,7 +22,11 @@
 namespace hexa {
 namespace {
 
+#ifndef HEXA_FACT_PF
+#define HEXA_FACT_PF 1
+#endif
 constexpr int BX = 16, BY = 16;
+constexpr int PF = HEXA_FACT_PF;   // layers prefetched ahead
 
 template <typename DT>
 __device__ __forceinline__ float dinv_f(DT v) { return (float)v; }
@@ -151,18 +155,17 @@ __global__ __launch_bounds__(BX* BY) void k_elastic_fact(Grid3 g, FactCoef<T> co
     }
     T carry[3] = {T(0), T(0), T(0)};
     T uown[3] = {lo[0][0], lo[0][1], lo[0][2]};
-    T nxt[3];
-    load_node(u, i, j, kb, g, nxt);
-    T fnext = (elem_col && (unsigned)(kb - 1) < (unsigned)g.nz) ? fac[ecol + plane_e * (kb - 1)] : T(0);
+    // software prefetch ring: the node plane and element factor of the next
+    // PF layers are in flight while the current layer is computed
+    T pu[PF][3], pf[PF];
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+        load_node(u, i, j, kb + d, g, pu[d]);
+        const int lay = kb - 1 + d;
+        pf[d] = (elem_col && (unsigned)lay < (unsigned)g.nz) ? fac[ecol + plane_e * lay] : T(0);
+    }
 
-    for (int kl = kb - 1; kl < ke; ++kl) {
-        // upper node plane kl+1 of element layer kl
-        T hi0[3] = {nxt[0], nxt[1], nxt[2]};
-        const T f = fnext;
-        // prefetch the next layer
-        load_node(u, i, j, kl + 2, g, nxt);
-        fnext = (elem_col && (unsigned)(kl + 1) < (unsigned)g.nz) ? fac[ecol + plane_e * (kl + 1)] : T(0);
-
+    auto layer = [&](const int kl, const T (&hi0)[3], const T f) {
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < 3; ++c) su[c][ty][tx] = hi0[c];
@@ -223,6 +226,21 @@ __global__ __launch_bounds__(BX* BY) void k_elastic_fact(Grid3 g, FactCoef<T> co
             for (int c = 0; c < 3; ++c) lo[q][c] = hi[q][c];
 #pragma unroll
         for (int c = 0; c < 3; ++c) uown[c] = hi0[c];
+    };
+
+    for (int k0 = kb - 1; k0 < ke; k0 += PF) {
+#pragma unroll
+        for (int d = 0; d < PF; ++d) {
+            const int kl = k0 + d;
+            if (kl >= ke) break;
+            // upper node plane kl+1 of element layer kl
+            const T hi0[3] = {pu[d][0], pu[d][1], pu[d][2]};
+            const T f = pf[d];
+            load_node(u, i, j, kl + PF + 1, g, pu[d]);
+            const int lay = kl + PF;
+            pf[d] = (elem_col && (unsigned)lay < (unsigned)g.nz) ? fac[ecol + plane_e * lay] : T(0);
+            layer(kl, hi0, f);
+        }
     }
 }
 

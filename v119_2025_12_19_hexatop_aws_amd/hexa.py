@@ -24,7 +24,7 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhexa.so")
+LIB_PATH = os.environ.get("HEXA_LIB_PATH") or os.path.join(_HERE, "libhexa.so")
 
 OK, E_INVALID, E_OOM, E_NAN_INPUT, E_NOT_CONVERGED, E_DEVICE = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_MIXED = 0, 1
