@@ -67,6 +67,39 @@ def cpu_baseline(p, sample_matvecs=2):
                       f"{dt * 1e3:.0f} ms each"}
 
 
+def serve(n_jobs, workers):
+    """Serving stream: Poisson arrivals (20/s, seeded), box meshes
+    log-uniform from 32^3 to 256x128x128 elements, aspect 1/2/4, support on a
+    random face, 1-4 point loads, volfrac U(0.2,0.4), 5 SIMP/OC iterations
+    each in mixed precision; admitted on one GPU by device-memory budget."""
+    from v119_2025_12_19_hexatop_aws_amd import hexa, problems as P
+    jobs = P.serving_stream(n_jobs)
+    free, total = __import__("torch").cuda.mem_get_info()
+    budget = int(free * 0.8)
+    log(f"serving {n_jobs} jobs, {workers} workers, budget {budget / 2**30:.0f} GiB")
+    with hexa.Server(workers=workers, mem_budget_bytes=budget) as s:
+        t0 = s.clock_ms()
+        ids = []
+        for j in jobs:
+            wait_s = j["arrival_s"] - (s.clock_ms() - t0) / 1e3
+            if wait_s > 0:
+                time.sleep(wait_s)
+            ids.append(s.submit(j["nx"], j["ny"], j["nz"], support_face=j["support_face"], n_loads=j["n_loads"],
+                                oc_iters=j["oc_iters"], precision=hexa.PREC_MIXED, volfrac=j["volfrac"],
+                                seed=j["seed"]))
+        res = [s.wait(i) for i in ids]
+    lat = np.array([r["t_end_ms"] - r["t_submit_ms"] for r in res])
+    solves = sum(r["oc_iters_done"] for r in res)
+    span = (max(r["t_end_ms"] for r in res) - min(r["t_submit_ms"] for r in res)) / 1e3
+    ok = sum(1 for r in res if r["status"] == 0)
+    out = dict(jobs=n_jobs, ok=ok, solves=solves, span_s=span, solves_per_s=solves / span,
+               jobs_per_s=n_jobs / span, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+               mean_elems=float(np.mean([j["nx"] * j["ny"] * j["nz"] for j in jobs])),
+               arrival_rate_per_s=20.0, workers=workers)
+    log("serving", out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +110,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--jacobi-max-iter", type=int, default=30000)
+    ap.add_argument("--serve-jobs", type=int, default=40, help="jobs of the serving stream (0 = skip)")
+    ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="also run one plain device copy of 8*nDOF bytes (the reference kernel that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -168,6 +203,10 @@ def main():
                 solves["speedup_mixed_vs_fp64_jacobi"] = (solves["jacobi_fp64"]["solve_s"]
                                                           / solves["mg_mixed"]["solve_s"])
 
+    serving = None
+    if args.serve_jobs > 0:
+        serving = serve(args.serve_jobs, args.serve_workers)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -193,6 +232,7 @@ def main():
                      "kernel_ms": ms_kernel},
         "matvec_fp32": {"kernel_ms": ms32, "GBps": nbytes32 / (ms32 * 1e-3) / 1e9},
         "solve": solves,
+        "serving": serving,
         "cpu_baseline": cpu,
         "device": dev["name"],
     }

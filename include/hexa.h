@@ -256,6 +256,45 @@ int32_t hexa_update_density(hexa_handle* h, float* density, const float* l1, con
                             const float* original_density, const hexa_update_params* p,
                             hexa_update_report* rep);
 
+/* ---- job server (no reference counterpart) ------------------------------
+ * Topology-optimisation jobs of mixed mesh sizes served on one GPU.  Each
+ * job: a box of nx*ny*nz unit bricks, all DOFs of one face held
+ * (support_face 0..5 = x0, x1, y0, y1, z0, z1), n_loads (1-4) unit point
+ * loads at seeded nodes of the opposite face with seeded directions, SIMP
+ * p=3, Emin=1e-9, uniform start rho = volfrac, oc_iters iterations of
+ * hexa_oc_step (MG-preconditioned solves to tol).  Workers take queued jobs
+ * first-fit when a worker is free and the job's estimated device bytes fit
+ * in the remaining budget (the admission rule generalises the VRAM guard of
+ * R/src/Utils/Helpers.jl:177-244); jobs run concurrently on their own
+ * streams.  Results do not depend on what else runs. */
+typedef struct hexa_server hexa_server;
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t support_face;
+    int32_t n_loads;
+    int32_t oc_iters;
+    int32_t precision;         /* HEXA_PREC_* */
+    int32_t pad;
+    double volfrac, move, rmin, tol;
+    uint64_t seed;
+} hexa_job;
+typedef struct {
+    int64_t id;
+    int32_t status, oc_iters_done;
+    int32_t solve_iterations, pad;
+    double compliance, volume;
+    double t_submit_ms, t_start_ms, t_end_ms;   /* server clock */
+    int64_t est_bytes, device_bytes;
+} hexa_job_result;
+/* Estimated device bytes of a handle on this grid (used for admission). */
+int64_t hexa_estimate_bytes(const hexa_grid* grid, const hexa_options* opt);
+int32_t hexa_server_create(int32_t workers, int64_t mem_budget_bytes, hexa_server** out);
+int32_t hexa_server_submit(hexa_server* s, const hexa_job* job, int64_t* id);
+/* Blocks until the job ends; x_out (nE) receives the final design or NULL. */
+int32_t hexa_server_wait(hexa_server* s, int64_t id, hexa_job_result* res, double* x_out);
+double hexa_server_clock_ms(hexa_server* s);
+int32_t hexa_server_destroy(hexa_server* s);
+
 #ifdef __cplusplus
 }
 #endif

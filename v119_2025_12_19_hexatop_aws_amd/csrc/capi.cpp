@@ -48,6 +48,7 @@ int32_t dalloc_bytes(hexa_handle* h, void** p, size_t bytes) {
         return HEXA_E_OOM;
     }
     h->allocs.push_back(*p);
+    h->bytes_allocated += (int64_t)(bytes < 16 ? 16 : bytes);
     return HEXA_OK;
 }
 
@@ -108,19 +109,17 @@ int32_t ensure_fac32(hexa_handle* h) {
 }
 
 // ------------------------------------------------------------------ MG
+// Number of levels: coarsen (ceil(n/2) per axis) until the coarsest grid has
+// at most COARSE_DOFS DOFs or is a single element, or to opt.mg_levels.
+constexpr int64_t COARSE_DOFS = 400;
 int choose_levels(const hexa_handle* h) {
     const int want = h->opt.mg_levels;
     int nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
     int L = 1;
-    if (want > 0) {
-        while (L < want && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 && nx >= 2 && ny >= 2 && nz >= 2) {
-            nx /= 2; ny /= 2; nz /= 2; ++L;
-        }
-        return L;
-    }
-    while (nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0 &&
-           3LL * (nx + 1) * (ny + 1) * (nz + 1) > 1200) {
-        nx /= 2; ny /= 2; nz /= 2; ++L;
+    auto dofs = [&] { return 3LL * (nx + 1) * (ny + 1) * (nz + 1); };
+    while ((want > 0 ? L < want : dofs() > COARSE_DOFS) && (nx > 1 || ny > 1 || nz > 1)) {
+        nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
+        ++L;
     }
     return L;
 }
@@ -135,7 +134,9 @@ int32_t mg_build(hexa_handle* h) {
     h->levels.assign(L, Level{});
     for (int l = 0; l < L; ++l) {
         Level& lv = h->levels[l];
-        lv.g = make_grid(h->g.nx >> l, h->g.ny >> l, h->g.nz >> l);
+        lv.g = l == 0 ? h->g
+                      : make_grid((h->levels[l - 1].g.nx + 1) / 2, (h->levels[l - 1].g.ny + 1) / 2,
+                                  (h->levels[l - 1].g.nz + 1) / 2);
         lv.kscale = std::ldexp(1.0, l);
         const int64_t nd = 3 * lv.g.nn;
         double ke[576];
@@ -157,7 +158,7 @@ int32_t mg_build(hexa_handle* h) {
             Level& up = h->levels[l - 1];
             HX_TRY(dalloc(h, &lv.fac64, (size_t)lv.g.ne));
             HX_TRY(dalloc(h, &lv.mask, (size_t)lv.g.nn));
-            launch_coarsen_factor<double>(h->stream, up.g, up.fac64, lv.fac64);
+            launch_coarsen_factor<double>(h->stream, up.g, lv.g, up.fac64, lv.fac64);
             launch_coarsen_mask(h->stream, up.g, lv.g, up.mask, lv.mask);
         }
         if (mx) {
@@ -185,7 +186,7 @@ int32_t mg_build(hexa_handle* h) {
     Level& c = h->levels[L - 1];
     h->n_coarse = 3 * c.g.nn;
     if (h->n_coarse > 20000) {
-        set_error("coarsest multigrid level has %lld DOFs (> 20000); use more levels",
+        set_error("coarsest multigrid level has %lld DOFs (> 20000); use more levels (mg_levels)",
                   (long long)h->n_coarse);
         return HEXA_E_INVALID;
     }

@@ -92,6 +92,7 @@ struct hexa_handle {
     int64_t* d_ties = nullptr;
 
     std::vector<void*> allocs;
+    int64_t bytes_allocated = 0;
 };
 
 namespace hexa {

@@ -113,7 +113,7 @@ void launch_diag(hipStream_t s, const Grid3& g, const double* Ke, const T* fac, 
 void launch_node_mask(hipStream_t s, const Grid3& g, const uint8_t* fixed_dof, const uint8_t* active,
                       uint8_t* mask);
 template <typename T>
-void launch_coarsen_factor(hipStream_t s, const Grid3& fine, const T* ffac, T* cfac);
+void launch_coarsen_factor(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* ffac, T* cfac);
 void launch_coarsen_mask(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
                          uint8_t* cmask);
 

@@ -214,32 +214,35 @@ __global__ void k_node_mask(Grid3 g, const uint8_t* __restrict__ fixed_dof,
     }
 }
 
-// Coarse stiffness multiplier: mean of the 8 children (rediscretisation).
+// Coarse stiffness multiplier (rediscretisation): sum of the 8 children / 8.
+// A coarse grid has ceil(n/2) elements per axis; on an odd fine axis the last
+// coarse element overhangs the box by one fine layer, whose missing children
+// count as void.
 template <typename T>
-__global__ void k_coarsen_factor(Grid3 f, const T* __restrict__ ffac, T* __restrict__ cfac) {
-    const int cx = f.nx / 2, cy = f.ny / 2, cz = f.nz / 2;
-    const int64_t nc = (int64_t)cx * cy * cz;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nc;
+__global__ void k_coarsen_factor(Grid3 f, Grid3 c, const T* __restrict__ ffac, T* __restrict__ cfac) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < c.ne;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const int I = (int)(e % cx), J = (int)((e / cx) % cy), K = (int)(e / ((int64_t)cx * cy));
+        const int I = (int)(e % c.nx), J = (int)((e / c.nx) % c.ny), K = (int)(e / ((int64_t)c.nx * c.ny));
         double s = 0;
-        for (int c = 0; c < 8; ++c) {
-            const int i = 2 * I + CX[c], j = 2 * J + CY[c], k = 2 * K + CZ[c];
-            s += (double)ffac[i + (int64_t)f.nx * (j + (int64_t)f.ny * k)];
+        for (int q = 0; q < 8; ++q) {
+            const int i = 2 * I + CX[q], j = 2 * J + CY[q], k = 2 * K + CZ[q];
+            if (i < f.nx && j < f.ny && k < f.nz) s += (double)ffac[i + (int64_t)f.nx * (j + (int64_t)f.ny * k)];
         }
         cfac[e] = (T)(0.125 * s);
     }
 }
 
-// Coarse node mask by injection: coarse node (I,J,K) takes the flags of
-// fine node (2I,2J,2K).
+// Coarse node mask by injection: coarse node (I,J,K) takes the flags of fine
+// node (2I,2J,2K), clamped to the box for the overhanging nodes of an odd
+// axis (so supports on a max face reach the coarse level).
 __global__ void k_coarsen_mask(Grid3 f, Grid3 c, const uint8_t* __restrict__ fmask,
                                uint8_t* __restrict__ cmask) {
     const int CNX = c.nx + 1, CNY = c.ny + 1, FNX = f.nx + 1, FNY = f.ny + 1;
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < c.nn;
          n += (int64_t)gridDim.x * blockDim.x) {
         const int I = (int)(n % CNX), J = (int)((n / CNX) % CNY), K = (int)(n / ((int64_t)CNX * CNY));
-        cmask[n] = fmask[2 * I + (int64_t)FNX * (2 * J + (int64_t)FNY * 2 * K)];
+        const int i = min(2 * I, f.nx), j = min(2 * J, f.ny), k = min(2 * K, f.nz);
+        cmask[n] = fmask[i + (int64_t)FNX * (j + (int64_t)FNY * k)];
     }
 }
 
@@ -292,9 +295,8 @@ void launch_node_mask(hipStream_t s, const Grid3& g, const uint8_t* fixed_dof, c
 }
 
 template <typename T>
-void launch_coarsen_factor(hipStream_t s, const Grid3& fine, const T* ffac, T* cfac) {
-    const int64_t nc = (int64_t)(fine.nx / 2) * (fine.ny / 2) * (fine.nz / 2);
-    k_coarsen_factor<T><<<grid_for(nc), 256, 0, s>>>(fine, ffac, cfac);
+void launch_coarsen_factor(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* ffac, T* cfac) {
+    k_coarsen_factor<T><<<grid_for(coarse.ne), 256, 0, s>>>(fine, coarse, ffac, cfac);
 }
 
 void launch_coarsen_mask(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
@@ -314,7 +316,7 @@ template void launch_stiffness_factor<float>(hipStream_t, int64_t, const double*
                                              double, double, double, float*);
 template void launch_diag<double>(hipStream_t, const Grid3&, const double*, const double*,
                                   const uint8_t*, int, double*);
-template void launch_coarsen_factor<double>(hipStream_t, const Grid3&, const double*, double*);
-template void launch_coarsen_factor<float>(hipStream_t, const Grid3&, const float*, float*);
+template void launch_coarsen_factor<double>(hipStream_t, const Grid3&, const Grid3&, const double*, double*);
+template void launch_coarsen_factor<float>(hipStream_t, const Grid3&, const Grid3&, const float*, float*);
 
 }  // namespace hexa

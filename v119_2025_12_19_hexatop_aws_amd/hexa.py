@@ -112,6 +112,23 @@ class UpdateReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Job(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("support_face", C.c_int32),
+                ("n_loads", C.c_int32), ("oc_iters", C.c_int32), ("precision", C.c_int32), ("pad", C.c_int32),
+                ("volfrac", C.c_double), ("move", C.c_double), ("rmin", C.c_double), ("tol", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("oc_iters_done", C.c_int32),
+                ("solve_iterations", C.c_int32), ("pad", C.c_int32), ("compliance", C.c_double),
+                ("volume", C.c_double), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_end_ms", C.c_double), ("est_bytes", C.c_int64), ("device_bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 _H = C.c_void_p
 _P = C.c_void_p
 
@@ -143,6 +160,12 @@ SIGNATURES = {
                                    C.POINTER(C.c_double)]),
     "hexa_oc_step": (C.c_int32, [_H, _P, C.POINTER(OcParams), C.POINTER(OcReport)]),
     "hexa_update_density": (C.c_int32, [_H, _P, _P, _P, _P, C.POINTER(UpdateParams), C.POINTER(UpdateReport)]),
+    "hexa_estimate_bytes": (C.c_int64, [C.POINTER(Grid), C.POINTER(Options)]),
+    "hexa_server_create": (C.c_int32, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
+    "hexa_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_int64)]),
+    "hexa_server_wait": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(JobResult), _P]),
+    "hexa_server_clock_ms": (C.c_double, [C.c_void_p]),
+    "hexa_server_destroy": (C.c_int32, [C.c_void_p]),
 }
 
 _lib = None
@@ -350,6 +373,46 @@ class Solver:
         p = C.c_void_p()
         _check(lib().hexa_displacement_dev(self._h, C.byref(p)))
         return p.value
+
+
+class Server:
+    """Device-resident job server (include/hexa.h, hexa_server_*)."""
+
+    def __init__(self, workers=8, mem_budget_bytes=200 << 30):
+        h = C.c_void_p()
+        _check(lib().hexa_server_create(int(workers), int(mem_budget_bytes), C.byref(h)))
+        self._s = h
+        self._ne = {}
+
+    def clock_ms(self):
+        return lib().hexa_server_clock_ms(self._s)
+
+    def submit(self, nx, ny, nz, support_face=0, n_loads=1, oc_iters=5, precision=PREC_MIXED, volfrac=0.3,
+               move=0.2, rmin=1.5, tol=1e-8, seed=0):
+        j = Job(nx=nx, ny=ny, nz=nz, support_face=support_face, n_loads=n_loads, oc_iters=oc_iters,
+                precision=precision, volfrac=volfrac, move=move, rmin=rmin, tol=tol, seed=seed)
+        i = C.c_int64()
+        _check(lib().hexa_server_submit(self._s, C.byref(j), C.byref(i)))
+        self._ne[i.value] = nx * ny * nz
+        return i.value
+
+    def wait(self, job_id, want_x=False):
+        r = JobResult()
+        x = np.zeros(self._ne.pop(job_id)) if want_x else None
+        lib().hexa_server_wait(self._s, int(job_id), C.byref(r), _ptr(x))
+        d = r.as_dict()
+        return (d, x) if want_x else d
+
+    def close(self):
+        if self._s:
+            lib().hexa_server_destroy(self._s)
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 # ---------------------------------------------------------------- mirrors

@@ -189,3 +189,23 @@ def cantilever_c4(nrhs=8) -> Problem:
     p.f = point_loads(256, 128, 128, nrhs, seed=4)
     p.meta.update(name="cantilever 256x128x128 x8 load cases")
     return p
+
+
+def serving_stream(n_jobs, seed=5, rate=20.0, min_elems=32 ** 3, max_elems=4194304, oc_iters=5, multiple=8):
+    """Seeded request stream: Poisson arrivals (``rate`` per second), element
+    counts log-uniform in [min_elems, max_elems], aspect nx:ny:nz = a:1:1 with
+    a in {1,2,4}, dimensions rounded to multiples of ``multiple`` (for the
+    multigrid hierarchy), support on a random face, 1-4 random point loads,
+    volfrac U(0.2, 0.4)."""
+    jobs, t = [], 0.0
+    for j in range(n_jobs):
+        u = uniform(seed, np.arange(16 * j, 16 * j + 8, dtype=np.uint64))
+        t += -np.log(1.0 - u[0]) / rate
+        ne = float(np.exp(np.log(min_elems) + u[1] * (np.log(max_elems) - np.log(min_elems))))
+        a = (1, 2, 4)[min(2, int(u[2] * 3))]
+        m = (ne / a) ** (1.0 / 3.0)
+        m = max(multiple, int(round(m / multiple)) * multiple)
+        jobs.append(dict(arrival_s=t, nx=a * m, ny=m, nz=m, support_face=min(5, int(u[3] * 6)),
+                         n_loads=1 + min(3, int(u[4] * 4)), volfrac=0.2 + 0.2 * u[5], oc_iters=oc_iters,
+                         seed=int(splitmix64(seed + 1000, j))))
+    return jobs
