@@ -228,7 +228,7 @@ def main():
                    "grid": [p.nx, p.ny, p.nz], "n_elem": p.n_elem, "n_dof": p.n_dof},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_elastic_fact<double>", "bytes_per_launch": nbytes64,
+                     "kernel": "k_elastic_rw<double>", "bytes_per_launch": nbytes64,
                      "kernel_ms": ms_kernel},
         "matvec_fp32": {"kernel_ms": ms32, "GBps": nbytes32 / (ms32 * 1e-3) / 1e9},
         "solve": solves,

@@ -85,12 +85,13 @@ int32_t poll_state(hexa_handle* h, int k, bool* halted) {
 
 namespace {
 
-// Elasticity operator on one grid: factored kernel for the exact brick
-// stiffness, dense-Ke kernel otherwise (e.g. the reference's f32-widened Ke).
+// Elasticity operator on one grid: factored row-wave kernel for the exact
+// brick stiffness, dense-Ke kernel otherwise (e.g. the reference's
+// f32-widened Ke).
 template <typename T, typename DT>
 void elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
              const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y) {
-    if (h->factored) launch_elastic_fact<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
+    if (h->factored) launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
     else launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
 }
 

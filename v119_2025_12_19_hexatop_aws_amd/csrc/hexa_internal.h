@@ -102,6 +102,9 @@ void launch_elastic_op(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, 
 template <typename T, typename DT>
 void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                          const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
+template <typename T, typename DT>
+void launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
+                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
 template <typename T>
 FactCoef<T> make_fact_coef(double hx, double hy, double hz, double nu);
 template <typename T>

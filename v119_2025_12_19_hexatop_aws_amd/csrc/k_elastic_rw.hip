@@ -1,0 +1,365 @@
+// Row-wave element operator for the exact brick stiffness (gfx950).
+//
+// Same factored element operator as k_elastic_fact.hip (per-axis sum /
+// difference basis: butterfly, modal diagonal + cross-component couplings,
+// transposed butterfly), mapped onto the machine differently:
+//
+//  * x runs along the 64 lanes of a wavefront.  A lane owns one node column
+//    and the element column to its +x side; neighbouring node values and the
+//    +x corner results move between lanes with DPP wave shifts, not LDS.
+//    Lanes 0 and 63 are halo: 62 owned columns per 64 loaded.
+//  * y runs across the BY waves of a block; the +y node row and the +y
+//    corner results are exchanged through LDS (3 values each per node and
+//    layer).  Waves 0 and BY-1 are halo rows; the last wave only loads.
+//  * z is marched, with the node planes and element factors of the next two
+//    layers in flight.  The butterfly is separable, so the x-y part of a
+//    node plane's transform is computed once and used by the element layers
+//    below and above it, and the two layers' x-y modal results for a plane
+//    are summed before one inverse x-y butterfly: per element about 175
+//    flops instead of 210.
+//  * memory is read and written through buffer descriptors (see below).
+//  * tiles are numbered so that each XCD (workgroups go to the 8 XCDs
+//    round-robin) processes a contiguous run of neighbouring tiles, whose
+//    shared halo rows then come from its own L2.
+//
+// Every node sums its contributions in a fixed order: no atomics and the same
+// result on every run.
+#include <algorithm>
+#include <climits>
+
+#include "hexa_internal.h"
+
+namespace hexa {
+namespace {
+
+#ifndef HEXA_RW_BY
+#define HEXA_RW_BY 8
+#endif
+constexpr int WL = 64;           // lanes per wavefront = node columns per tile
+constexpr int OWN_X = WL - 2;    // owned node columns per tile
+constexpr int BY = HEXA_RW_BY;   // wavefronts per block = node rows per tile
+constexpr int OWN_Y = BY - 2;    // owned node rows per tile
+constexpr int NXCD = 8;
+
+// DPP wave shifts (GFX9 encodings): lane l reads lane l+1 / lane l-1; the
+// lane without a source reads 0.
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
+
+__device__ __forceinline__ float from_next(float v) { return __int_as_float(dpp_next(__float_as_int(v))); }
+__device__ __forceinline__ float from_prev(float v) { return __int_as_float(dpp_prev(__float_as_int(v))); }
+__device__ __forceinline__ double from_next(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_next((int)(b & 0xffffffffLL)), hi = dpp_next((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double from_prev(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_prev((int)(b & 0xffffffffLL)), hi = dpp_prev((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Buffer (resource-descriptor) memory access: a lane whose offset lies past
+// the descriptor's byte count reads 0 and its store is dropped, so halo lanes
+// and out-of-box planes need no branches, and every layer issues the same
+// memory instructions (the compiler can then keep prefetches in flight
+// across the layer's barriers).
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t OOB = 0x80000000u;   // > any descriptor's byte count
+
+__device__ __forceinline__ Rsrc rsrc(const void* p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void ld3(Rsrc r, uint32_t off, double (&o)[3]) {
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0);
+    o[0] = __builtin_bit_cast(double, u32x2{a.x, a.y});
+    o[1] = __builtin_bit_cast(double, u32x2{a.z, a.w});
+    o[2] = __builtin_bit_cast(double, b);
+}
+__device__ __forceinline__ void ld3(Rsrc r, uint32_t off, float (&o)[3]) {
+    const u32x3 a = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+    o[0] = __uint_as_float(a.x);
+    o[1] = __uint_as_float(a.y);
+    o[2] = __uint_as_float(a.z);
+}
+__device__ __forceinline__ void st3(Rsrc r, uint32_t off, const double (&o)[3]) {
+    const u32x2 a = __builtin_bit_cast(u32x2, o[0]), b = __builtin_bit_cast(u32x2, o[1]),
+                c = __builtin_bit_cast(u32x2, o[2]);
+    __builtin_amdgcn_raw_buffer_store_b128(u32x4{a.x, a.y, b.x, b.y}, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(c, r, off + 16, 0, 0);
+}
+__device__ __forceinline__ void st3(Rsrc r, uint32_t off, const float (&o)[3]) {
+    __builtin_amdgcn_raw_buffer_store_b96(u32x3{__float_as_uint(o[0]), __float_as_uint(o[1]), __float_as_uint(o[2])},
+                                          r, off, 0, 0);
+}
+__device__ __forceinline__ double ld1(Rsrc r, uint32_t off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float ld1(Rsrc r, uint32_t off, float) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void ld3_dinv(Rsrc r, uint32_t off, const double*, T (&o)[3]) {
+    double d[3];
+    ld3(r, off, d);
+    for (int c = 0; c < 3; ++c) o[c] = (T)d[c];
+}
+template <typename T>
+__device__ __forceinline__ void ld3_dinv(Rsrc r, uint32_t off, const float*, T (&o)[3]) {
+    float d[3];
+    ld3(r, off, d);
+    for (int c = 0; c < 3; ++c) o[c] = (T)d[c];
+}
+template <typename T>
+__device__ __forceinline__ void ld3_dinv(Rsrc r, uint32_t off, const uint16_t*, T (&o)[3]) {
+    for (int c = 0; c < 3; ++c)
+        o[c] = (T)__uint_as_float(((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off + 2 * c, 0, 0)) << 16);
+}
+
+template <typename T, typename DT, int MODE>
+__global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef, const T* __restrict__ fac,
+                                                       const uint8_t* __restrict__ mask,
+                                                       const T* __restrict__ u, const T* __restrict__ b,
+                                                       const DT* __restrict__ dinv, T omega, T shift,
+                                                       T* __restrict__ y, int kchunk, int tiles_x, int tiles_y,
+                                                       int tiles) {
+    __shared__ T su[BY][3][WL];   // node values of each wave's row, current plane
+    __shared__ T sc[BY][3][WL];   // each wave's results for the row above it
+
+    const int l = threadIdx.x, w = threadIdx.y;
+    // XCD-contiguous tile order: workgroup L runs on XCD L % 8
+    const int L = blockIdx.x;
+    const int xcd = L % NXCD, q = tiles / NXCD, r = tiles % NXCD;
+    const int t = xcd * q + min(xcd, r) + L / NXCD;
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
+
+    const int i = tx * OWN_X - 1 + l;
+    const int j = ty * OWN_Y - 1 + w;
+    const int kb = tz * kchunk;
+    const int ke = min(kb + kchunk, g.nz + 1);
+    // the last wave only supplies its node row to the wave below
+    const bool compute = w < BY - 1;
+    const bool col_in = (unsigned)i <= (unsigned)g.nx && (unsigned)j <= (unsigned)g.ny;
+    const bool elem = l < WL - 1 && compute && (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const bool owner = l >= 1 && l <= WL - 2 && w >= 1 && w <= BY - 2 && col_in;
+    const int64_t plane_n = (int64_t)(g.nx + 1) * (g.ny + 1);
+    const int64_t plane_e = (int64_t)g.nx * g.ny;
+
+    // loop-invariant lane offsets (bytes); OOB for lanes that must not touch memory
+    const int64_t nrow = i + (int64_t)(g.nx + 1) * j;
+    const uint32_t uoff = col_in ? (uint32_t)(nrow * 3 * sizeof(T)) : OOB;
+    const uint32_t foff = elem ? (uint32_t)((i + (int64_t)g.nx * j) * sizeof(T)) : OOB;
+    const uint32_t ooff = owner ? (uint32_t)(nrow * 3 * sizeof(T)) : OOB;
+    const uint32_t moff = owner ? (uint32_t)nrow : OOB;
+    const uint32_t doff = owner ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
+    const int64_t vplane = 3 * plane_n;   // vector entries per node plane
+
+    auto load_own = [&](int k, T (&o)[3]) {
+        const bool in = (unsigned)k <= (unsigned)g.nz;
+        ld3(rsrc(u + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0), uoff, o);
+    };
+    auto load_fac = [&](int lay) -> T {
+        const bool in = (unsigned)lay < (unsigned)g.nz;
+        return ld1(rsrc(fac + (in ? plane_e * lay : 0), in ? plane_e * (int64_t)sizeof(T) : 0), foff, T(0));
+    };
+    // x-y modes of the element column's 4 corners in one node plane, from
+    // this lane's node value: P[c][mx + 2 my].  Contains a barrier.
+    auto plane_modes = [&](const T (&own)[3], T (&P)[3][4]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) su[w][c][l] = own[c];
+        __syncthreads();
+        if (compute) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T up = su[w + 1][c][l];
+                const T a = own[c], bx = from_next(own[c]), cy = up, d = from_next(up);
+                const T s0 = a + bx, d0 = bx - a, s1 = cy + d, d1 = d - cy;
+                P[c][0] = s0 + s1;
+                P[c][1] = d0 + d1;
+                P[c][2] = s1 - s0;
+                P[c][3] = d1 - d0;
+            }
+        }
+    };
+
+    // One element layer kl (node planes kl, kl+1) and the output of node
+    // plane kl.  Plo/Yin: x-y modes of plane kl and the layer below's modal
+    // results for plane kl; Phi/Yout: the same for plane kl+1.
+    struct Out {
+        Rsrc ry;
+        uint32_t mb;
+        T bv[3], dv[3];
+    };
+    // flags, right-hand side and inverse diagonal of output plane kl (the
+    // chunk's first layer, kl = kb - 1, has none); issued before the layer's
+    // prefetches, so waiting for them leaves the prefetches in flight
+    auto load_out = [&](const int kl, Out& o) {
+        const bool has_out = kl >= kb;
+        const int64_t ko = has_out ? kl : 0;
+        o.ry = rsrc(y + vplane * ko, has_out ? vplane * (int64_t)sizeof(T) : 0);
+        // (a null mask gives an empty descriptor: all flags read 0)
+        o.mb = __builtin_amdgcn_raw_buffer_load_b8(
+            rsrc(mask ? mask + plane_n * ko : mask, (mask && has_out) ? plane_n : 0), moff, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o.bv[c] = o.dv[c] = T(0);
+        if (MODE != OP_APPLY) ld3(rsrc(b + vplane * ko, has_out ? vplane * (int64_t)sizeof(T) : 0), ooff, o.bv);
+        if (MODE == OP_SMOOTH)
+            ld3_dinv(rsrc(dinv + vplane * ko, has_out ? vplane * (int64_t)sizeof(DT) : 0), doff, dinv, o.dv);
+    };
+    auto step = [&](const T (&Plo)[3][4], T (&Phi)[3][4], const T (&Yin)[3][4], T (&Yout)[3][4],
+                    const T (&ulo)[3], const T (&uhi)[3], const T fl, const Out& ot) {
+        plane_modes(uhi, Phi);   // barrier A
+        T N0[3] = {T(0), T(0), T(0)};
+        if (compute) {
+            // modes v[c][m], m = mx + 2my + 4mz (mode 0 is not used)
+            T v[3][8];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (m) v[c][m] = Plo[c][m] + Phi[c][m];
+                    v[c][m + 4] = Phi[c][m] - Plo[c][m];
+                }
+            T N1[3];
+            // one output component at a time keeps few modal results live
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T yv[8];
+#pragma unroll
+                for (int m = 1; m < 8; ++m) yv[m] = coef.D[c][m] * v[c][m];
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb) {
+                    if (bb == c) continue;
+                    const int d = 3 - c - bb;
+#pragma unroll
+                    for (int qq = 0; qq < 2; ++qq) {
+                        const int mq = qq << d;
+                        yv[(1 << c) | mq] += coef.cl[d][qq] * v[bb][(1 << bb) | mq];
+                        yv[(1 << bb) | mq] += coef.cm[d][qq] * v[bb][(1 << c) | mq];
+                    }
+                }
+                // element factor and the z part of the transposed butterfly:
+                // plane kl gets (y_m - y_m+4), plane kl+1 gets (y_m + y_m+4)
+                T zlo[4];
+                const T e4 = fl * yv[4];
+                zlo[0] = Yin[c][0] - e4;
+                Yout[c][0] = e4;
+#pragma unroll
+                for (int m = 1; m < 4; ++m) {
+                    const T e0 = fl * yv[m], e1 = fl * yv[m + 4];
+                    zlo[m] = Yin[c][m] + (e0 - e1);
+                    Yout[c][m] = e0 + e1;
+                }
+                // x-y part of the transposed butterfly -> corners cx + 2cy
+                const T t0 = zlo[0] - zlo[1], t1 = zlo[0] + zlo[1], t2 = zlo[2] - zlo[3], t3 = zlo[2] + zlo[3];
+                const T c00 = t0 - t2, c10 = t1 - t3, c01 = t0 + t2, c11 = t1 + t3;
+                // node (i, j) takes corner (0,0) of its element and (1,0) of
+                // the element of lane l-1; likewise for the row above
+                N0[c] = c00 + from_prev(c10);
+                N1[c] = c01 + from_prev(c11);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sc[w][c][l] = N1[c];
+        }
+        __syncthreads();   // barrier B
+        T o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T below = sc[w > 0 ? w - 1 : 0][c][l];
+            const T Au = N0[c] + (owner ? below : T(0)) + shift * ulo[c];
+            T v;
+            if (MODE == OP_APPLY) v = Au;
+            else if (MODE == OP_RESID) v = ot.bv[c] - Au;
+            else v = ulo[c] + omega * ot.dv[c] * (ot.bv[c] - Au);
+            o[c] = ((ot.mb >> c) & 1) ? T(0) : v;
+        }
+        st3(ot.ry, ooff, o);
+    };
+
+    T uA[3], uB[3], PA[3][4], PB[3][4], YA[3][4], YB[3][4];
+    load_own(kb - 1, uA);
+    plane_modes(uA, PA);
+    __syncthreads();   // su is rewritten by the first layer
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) YA[c][m] = T(0);
+    // two-deep prefetch ring: planes kl+1, kl+2 and factors of layers kl, kl+1
+    T r0[3], r1[3];
+    load_own(kb, r0);
+    load_own(kb + 1, r1);
+    T f0 = load_fac(kb - 1), f1 = load_fac(kb);
+
+    for (int kl = kb - 1; kl < ke; kl += 2) {
+        {
+            Out ot;
+            load_out(kl, ot);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) uB[c] = r0[c];
+            const T fl = f0;
+            load_own(kl + 3, r0);
+            f0 = load_fac(kl + 2);
+            step(PA, PB, YA, YB, uA, uB, fl, ot);
+        }
+        if (kl + 1 >= ke) break;
+        {
+            Out ot;
+            load_out(kl + 1, ot);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) uA[c] = r1[c];
+            const T fl = f1;
+            load_own(kl + 4, r1);
+            f1 = load_fac(kl + 3);
+            step(PB, PA, YB, YA, uB, uA, fl, ot);
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T, typename DT>
+void launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
+                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y) {
+    // buffer descriptors address one node plane (byte counts are 31-bit)
+    if (3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)std::max(sizeof(T), sizeof(DT)) > INT32_MAX) {
+        launch_elastic_fact<T, DT>(s, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
+        return;
+    }
+    const int tiles_x = (g.nx + 1 + OWN_X - 1) / OWN_X;
+    const int tiles_y = (g.ny + 1 + OWN_Y - 1) / OWN_Y;
+    // node planes per block: enough blocks to fill the chip, long enough
+    // chunks to amortise the one recomputed element layer per block
+    int kchunk = 32;
+    while (kchunk > 4 && (int64_t)tiles_x * tiles_y * ((g.nz + 1 + kchunk - 1) / kchunk) < 2048) kchunk /= 2;
+    const int tiles_z = (g.nz + 1 + kchunk - 1) / kchunk;
+    const int tiles = tiles_x * tiles_y * tiles_z;
+    dim3 block(WL, BY);
+    switch (mode) {
+        case OP_APPLY:
+            k_elastic_rw<T, DT, OP_APPLY><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
+                                                                 kchunk, tiles_x, tiles_y, tiles);
+            break;
+        case OP_RESID:
+            k_elastic_rw<T, DT, OP_RESID><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
+                                                                 kchunk, tiles_x, tiles_y, tiles);
+            break;
+        default:
+            k_elastic_rw<T, DT, OP_SMOOTH><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
+                                                                  kchunk, tiles_x, tiles_y, tiles);
+            break;
+    }
+}
+
+#define HX_INST_RW(T, DT)                                                                             \
+    template void launch_elastic_rw<T, DT>(hipStream_t, const Grid3&, OpMode, const FactCoef<T>&,     \
+                                           const T*, const uint8_t*, const T*, const T*, const DT*, T, \
+                                           T, T*);
+HX_INST_RW(double, double)
+HX_INST_RW(float, float)
+HX_INST_RW(float, uint16_t)
+
+}  // namespace hexa
