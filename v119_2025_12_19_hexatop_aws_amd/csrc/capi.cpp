@@ -88,11 +88,16 @@ namespace {
 // Elasticity operator on one grid: factored row-wave kernel for the exact
 // brick stiffness, dense-Ke kernel otherwise (e.g. the reference's
 // f32-widened Ke).
+// With dot_part, the factored kernel also writes per-block partial sums of
+// (u, y); the return value is their count (0: not provided).
 template <typename T, typename DT>
-void elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
-             const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y) {
-    if (h->factored) launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
-    else launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
+int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
+            const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
+            double* dot_part = nullptr) {
+    if (h->factored)
+        return launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y, dot_part);
+    launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
+    return 0;
 }
 
 int32_t ensure_fac32(hexa_handle* h) {
@@ -325,18 +330,33 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
 
     int k_done = 0;
     bool stop = false;
+    // mixed V-cycle: the update writes its FP32 input and the dots and the
+    // p-update read its FP32 output, so no FP64 copy of z is formed
+    const bool mx = use_mg && mixed(h) && h->levels.size() > 1;
+    float* r32 = mx ? (float*)h->levels[0].b : nullptr;
+    const float* z32 = mx ? (const float*)h->levels[0].x : nullptr;
     for (int k = 1; k <= prm->max_iter && !stop; ++k) {
-        elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask, h->d_p,
-                                nullptr, nullptr, 0.0, s, h->d_Ap);
+        // Ap = (K + sI) p, with the per-block partials of (p, Ap)
+        const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask,
+                                               h->d_p, nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart);
         ++*matvecs;
-        launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
-        launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
+        if (nq > 0) {
+            launch_pcg_alpha_n(st, h->d_state, h->d_dotpart, nq, cpu ? 0 : 1);
+        } else {
+            launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
+            launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
+        }
         launch_pcg_update(st, nd, h->d_state, h->d_x, h->d_p, h->d_r, h->d_Ap, use_mg ? nullptr : h->d_minv,
-                          h->d_partials);
+                          h->d_partials, r32);
         const int check = cpu ? 1 : ((k % check_every == 0) || k == 1);
         launch_pcg_check(st, h->d_state, h->d_partials, k, check, cpu ? 1 : 0, use_mg ? 0 : 1);
         if (check && !cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
-        if (use_mg) {
+        if (mx) {
+            vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32);
+            launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_partials);
+            launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32);
+        } else if (use_mg) {
             apply_mg(h, h->d_r, h->d_z);
             launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);
             launch_pcg_flex_beta(st, h->d_state, h->d_partials);
@@ -519,6 +539,7 @@ int32_t ensure_solver_vectors(hexa_handle* h) {
     HX_TRY(dalloc(h, &h->d_b, nd));
     HX_TRY(dalloc(h, &h->d_diag, nd));
     HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
+    HX_TRY(dalloc(h, &h->d_dotpart, (size_t)std::max<int64_t>(1, elastic_rw_blocks(h->g))));
     HX_TRY(dalloc(h, &h->d_scal, 16));
     HX_TRY(dalloc(h, &h->d_state, 1));
     return HEXA_OK;

@@ -65,6 +65,7 @@ struct hexa_handle {
     double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_z = nullptr;
     double *d_minv = nullptr, *d_best = nullptr, *d_b = nullptr, *d_diag = nullptr, *d_guess = nullptr;
     double* d_partials = nullptr;
+    double* d_dotpart = nullptr;   // per-block (p, Ap) partials of the fused operator
     double* d_scal = nullptr;
     hexa::PcgState* d_state = nullptr;
     hexa::PcgState* h_state = nullptr;   // pinned ring

@@ -102,9 +102,14 @@ void launch_elastic_op(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, 
 template <typename T, typename DT>
 void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                          const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
+// Returns the number of per-block (u, y) partial sums written to dot_part
+// (0 when dot_part is null or the kernel cannot provide them).
 template <typename T, typename DT>
-void launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
-                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
+int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
+                      const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
+                      double* dot_part = nullptr);
+// Upper bound on the partial sums launch_elastic_rw writes for grid g.
+int64_t elastic_rw_blocks(const Grid3& g);
 template <typename T>
 FactCoef<T> make_fact_coef(double hx, double hy, double hz, double nu);
 template <typename T>
@@ -189,7 +194,7 @@ void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, con
                     double* partials);
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check);
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials);
+                       const double* Ap, const double* minv, double* partials, float* r32 = nullptr);
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
                       int cpu_semantics, int jacobi);
 void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
@@ -197,6 +202,10 @@ void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const do
 void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials);
 void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
                          const double* r, const double* minv);
+void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int n, int breakdown_check);
+void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
+                              const double* Ap, double* partials);
+void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z);
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best);
 
 }  // namespace hexa

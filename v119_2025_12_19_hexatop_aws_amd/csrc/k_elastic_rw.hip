@@ -126,7 +126,7 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
                                                        const T* __restrict__ u, const T* __restrict__ b,
                                                        const DT* __restrict__ dinv, T omega, T shift,
                                                        T* __restrict__ y, int kchunk, int tiles_x, int tiles_y,
-                                                       int tiles) {
+                                                       int tiles, double* __restrict__ dot_part) {
     __shared__ T su[BY][3][WL];   // node values of each wave's row, current plane
     __shared__ T sc[BY][3][WL];   // each wave's results for the row above it
 
@@ -192,14 +192,18 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
     struct Out {
         Rsrc ry;
         uint32_t mb;
+        bool has_out;
         T bv[3], dv[3];
     };
+    // this lane's share of (u, y) over its owned outputs, when dot_part is set
+    double acc = 0.0;
     // flags, right-hand side and inverse diagonal of output plane kl (the
     // chunk's first layer, kl = kb - 1, has none); issued before the layer's
     // prefetches, so waiting for them leaves the prefetches in flight
     auto load_out = [&](const int kl, Out& o) {
         const bool has_out = kl >= kb;
         const int64_t ko = has_out ? kl : 0;
+        o.has_out = has_out;
         o.ry = rsrc(y + vplane * ko, has_out ? vplane * (int64_t)sizeof(T) : 0);
         // (a null mask gives an empty descriptor: all flags read 0)
         o.mb = __builtin_amdgcn_raw_buffer_load_b8(
@@ -278,6 +282,7 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
             o[c] = ((ot.mb >> c) & 1) ? T(0) : v;
         }
         st3(ot.ry, ooff, o);
+        if (dot_part && ot.has_out && owner) acc += (double)ulo[0] * o[0] + (double)ulo[1] * o[1] + (double)ulo[2] * o[2];
     };
 
     T uA[3], uB[3], PA[3][4], PB[3][4], YA[3][4], YB[3][4];
@@ -317,47 +322,80 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
             step(PB, PA, YB, YA, uB, uA, fl, ot);
         }
     }
+    if (dot_part) {
+        // fixed-order block sum: lanes by butterfly, then waves in order
+        double v = acc;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        __syncthreads();
+        double* red = reinterpret_cast<double*>(&su[0][0][0]);
+        if (l == 0) red[w] = v;
+        __syncthreads();
+        if (l == 0 && w == 0) {
+            double tsum = 0.0;
+            for (int q = 0; q < BY; ++q) tsum += red[q];
+            dot_part[L] = tsum;
+        }
+    }
 }
 
 }  // namespace
 
+namespace {
+struct RwShape {
+    int tiles_x, tiles_y, tiles_z, kchunk;
+};
+RwShape rw_shape(const Grid3& g) {
+    RwShape r;
+    r.tiles_x = (g.nx + 1 + OWN_X - 1) / OWN_X;
+    r.tiles_y = (g.ny + 1 + OWN_Y - 1) / OWN_Y;
+    // node planes per block: enough blocks to fill the chip, long enough
+    // chunks to amortise the one recomputed element layer per block
+    r.kchunk = 32;
+    while (r.kchunk > 4 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
+        r.kchunk /= 2;
+    r.tiles_z = (g.nz + 1 + r.kchunk - 1) / r.kchunk;
+    return r;
+}
+}  // namespace
+
+int64_t elastic_rw_blocks(const Grid3& g) {
+    const RwShape r = rw_shape(g);
+    return (int64_t)r.tiles_x * r.tiles_y * r.tiles_z;
+}
+
 template <typename T, typename DT>
-void launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
-                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y) {
+int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
+                      const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
+                      double* dot_part) {
     // buffer descriptors address one node plane (byte counts are 31-bit)
     if (3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)std::max(sizeof(T), sizeof(DT)) > INT32_MAX) {
         launch_elastic_fact<T, DT>(s, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
-        return;
+        return 0;
     }
-    const int tiles_x = (g.nx + 1 + OWN_X - 1) / OWN_X;
-    const int tiles_y = (g.ny + 1 + OWN_Y - 1) / OWN_Y;
-    // node planes per block: enough blocks to fill the chip, long enough
-    // chunks to amortise the one recomputed element layer per block
-    int kchunk = 32;
-    while (kchunk > 4 && (int64_t)tiles_x * tiles_y * ((g.nz + 1 + kchunk - 1) / kchunk) < 2048) kchunk /= 2;
-    const int tiles_z = (g.nz + 1 + kchunk - 1) / kchunk;
-    const int tiles = tiles_x * tiles_y * tiles_z;
+    const RwShape r = rw_shape(g);
+    const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
     dim3 block(WL, BY);
     switch (mode) {
         case OP_APPLY:
             k_elastic_rw<T, DT, OP_APPLY><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                 kchunk, tiles_x, tiles_y, tiles);
+                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
             break;
         case OP_RESID:
             k_elastic_rw<T, DT, OP_RESID><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                 kchunk, tiles_x, tiles_y, tiles);
+                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
             break;
         default:
             k_elastic_rw<T, DT, OP_SMOOTH><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                  kchunk, tiles_x, tiles_y, tiles);
+                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
             break;
     }
+    return dot_part ? tiles : 0;
 }
 
-#define HX_INST_RW(T, DT)                                                                             \
-    template void launch_elastic_rw<T, DT>(hipStream_t, const Grid3&, OpMode, const FactCoef<T>&,     \
-                                           const T*, const uint8_t*, const T*, const T*, const DT*, T, \
-                                           T, T*);
+#define HX_INST_RW(T, DT)                                                                            \
+    template int launch_elastic_rw<T, DT>(hipStream_t, const Grid3&, OpMode, const FactCoef<T>&,     \
+                                          const T*, const uint8_t*, const T*, const T*, const DT*, T, \
+                                          T, T*, double*);
 HX_INST_RW(double, double)
 HX_INST_RW(float, float)
 HX_INST_RW(float, uint16_t)

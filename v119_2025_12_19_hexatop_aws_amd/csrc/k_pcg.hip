@@ -37,6 +37,12 @@ __device__ __forceinline__ double final_sum(const double* __restrict__ partials,
     return block_sum(s, lds);
 }
 
+__device__ __forceinline__ double final_sum_n(const double* __restrict__ partials, int n, double* lds) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += RED_THREADS) s += partials[i];
+    return block_sum(s, lds);
+}
+
 // partials[0] holds ||b||^2 on entry.
 __global__ __launch_bounds__(RED_THREADS) void k_init_state(PcgState* st, const double* partials,
                                                             double tol) {
@@ -102,13 +108,32 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const doubl
     }
 }
 
+// alpha from n per-block partials of (p, Ap) written by the operator kernel.
+__global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const double* partials, int n,
+                                                         int breakdown_check) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double den = final_sum_n(partials, n, lds);
+    if (threadIdx.x == 0) {
+        st->pAp = den;
+        if (breakdown_check && fabs(den) < 1e-20) {
+            st->halted = PCG_BREAKDOWN;
+            st->alpha = 0.0;
+        } else {
+            st->alpha = st->rz / den;
+        }
+    }
+}
+
 // x += alpha p; r -= alpha Ap; partials: (r,r) and, when minv is given,
-// (r, minv r) for the Jacobi preconditioner.
+// (r, minv r) for the Jacobi preconditioner.  r32, when given, receives the
+// new residual in FP32 (the mixed V-cycle's input).
 __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, const PcgState* st, double* __restrict__ x,
                                                         const double* __restrict__ p, double* __restrict__ r,
                                                         const double* __restrict__ Ap,
                                                         const double* __restrict__ minv,
-                                                        double* __restrict__ partials) {
+                                                        double* __restrict__ partials,
+                                                        float* __restrict__ r32) {
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double a = st->alpha;
@@ -117,6 +142,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, const PcgStat
         x[i] += a * p[i];
         const double ri = r[i] - a * Ap[i];
         r[i] = ri;
+        if (r32) r32[i] = (float)ri;
         rr += ri * ri;
         if (minv) rz += ri * (ri * minv[i]);
     }
@@ -199,6 +225,37 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_beta(PcgState* st, const d
     }
 }
 
+// The same two dots with the correction z in FP32 (the mixed V-cycle's
+// output, widened exactly).
+__global__ __launch_bounds__(RED_THREADS) void k_flex_dots_z32(int64_t n, const PcgState* st,
+                                                               const double* __restrict__ r,
+                                                               const float* __restrict__ z,
+                                                               const double* __restrict__ Ap,
+                                                               double* __restrict__ partials) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    double rz = 0.0, zap = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const double zi = (double)z[i];
+        rz += r[i] * zi;
+        zap += zi * Ap[i];
+    }
+    rz = block_sum(rz, lds);
+    zap = block_sum(zap, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = rz;
+        partials[RED_BLOCKS + blockIdx.x] = zap;
+    }
+}
+
+__global__ void k_p_update_z32(int64_t n, const PcgState* st, double* __restrict__ p, const float* __restrict__ z) {
+    if (st->halted) return;
+    const double b = st->beta;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (double)z[i] + b * p[i];
+}
+
 // p = z + beta p; with z == nullptr, z = minv * r (Jacobi, not stored).
 __global__ void k_p_update(int64_t n, const PcgState* st, double* __restrict__ p, const double* __restrict__ z,
                            const double* __restrict__ r, const double* __restrict__ minv) {
@@ -237,9 +294,12 @@ void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, con
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check) {
     k_alpha<<<1, RED_THREADS, 0, s>>>(st, partials, breakdown_check);
 }
+void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int n, int breakdown_check) {
+    k_alpha_n<<<1, RED_THREADS, 0, s>>>(st, partials, n, breakdown_check);
+}
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials) {
-    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials);
+                       const double* Ap, const double* minv, double* partials, float* r32) {
+    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32);
 }
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
                       int cpu_semantics, int jacobi) {
@@ -255,6 +315,13 @@ void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials) {
 void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
                          const double* r, const double* minv) {
     k_p_update<<<grid_for(n), 256, 0, s>>>(n, st, p, z, r, minv);
+}
+void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
+                              const double* Ap, double* partials) {
+    k_flex_dots_z32<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials);
+}
+void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z) {
+    k_p_update_z32<<<grid_for(n), 256, 0, s>>>(n, st, p, z);
 }
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best) {
     k_best_copy<<<grid_for(n), 256, 0, s>>>(n, st, x, best);
