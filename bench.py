@@ -100,6 +100,51 @@ def serve(n_jobs, workers):
     return out
 
 
+def shard_shape(p, steps, warmup, parts=8):
+    """One GPU's share of an 8-way split of the grid into z-slabs (the
+    marched axis): the operator on nz/8 + 1 element layers (one halo layer),
+    and the contiguous halo node plane each neighbour exchange moves."""
+    import torch
+    from v119_2025_12_19_hexatop_aws_amd import hexa
+    nzs = p.nz // parts + 1
+    s = hexa.Solver(p.nx, p.ny, nzs, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu)
+    ne = p.nx * p.ny * nzs
+    s.set_density(p.rho[:ne])
+    nd = 3 * (p.nx + 1) * (p.ny + 1) * (nzs + 1)
+    x = torch.randn(nd, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    s.matvec_dev(x.data_ptr(), y.data_ptr(), hexa.PREC_FP64, reps=max(1, warmup))
+    ms = s.matvec_dev(x.data_ptr(), y.data_ptr(), hexa.PREC_FP64, reps=steps)
+    s.close()
+    nbytes = 8 * ne + 16 * nd
+    plane = 3 * 8 * (p.nx + 1) * (p.ny + 1)
+    out = dict(grid=[p.nx, p.ny, nzs], parts=parts, kernel_ms=ms, GBps=nbytes / (ms * 1e-3) / 1e9,
+               halo_plane_bytes=plane, halo_us_at_153GBps=plane / 153e9 * 1e6)
+    log("shard", out)
+    return out
+
+
+def load_cases(nrhs=8):
+    """8 load cases on the 256x128x128 cantilever (unit point loads on the
+    top/end faces, seeded): one handle, one multigrid hierarchy, mixed
+    precision to 1e-8 each."""
+    from v119_2025_12_19_hexatop_aws_amd import hexa, problems as P
+    p = P.cantilever_c4(nrhs)
+    s = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu, precision=hexa.PREC_MIXED, nrhs=nrhs)
+    s.set_problem(p.fixed, p.f.ravel())
+    s.set_density(p.rho)
+    t0 = time.perf_counter()
+    u, reps, st = s.solve(tol=1e-8, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2000)
+    wall = time.perf_counter() - t0
+    s.close()
+    out = dict(grid=list(p.grid), nrhs=nrhs, status=st, wall_s=wall,
+               solve_s=sum(r["solve_ms"] for r in reps) / 1e3, setup_s=sum(r["setup_ms"] for r in reps) / 1e3,
+               iterations=[r["iterations"] for r in reps], max_true_rel_res=max(r["true_rel_res"] for r in reps),
+               solves_per_s=nrhs / wall)
+    log("load cases", out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +157,7 @@ def main():
     ap.add_argument("--jacobi-max-iter", type=int, default=30000)
     ap.add_argument("--serve-jobs", type=int, default=40, help="jobs of the serving stream (0 = skip)")
     ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--no-load-cases", action="store_true", help="skip the 8-load-case solve")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="also run one plain device copy of 8*nDOF bytes (the reference kernel that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -160,6 +206,7 @@ def main():
         torch.cuda.synchronize()
     del x, y, x32, y32
     torch.cuda.empty_cache()
+    shard = None if args.traffic_probe else shard_shape(p, args.steps, args.warmup)
 
     solves = {}
     if not args.no_solve:
@@ -203,6 +250,10 @@ def main():
                 solves["speedup_mixed_vs_fp64_jacobi"] = (solves["jacobi_fp64"]["solve_s"]
                                                           / solves["mg_mixed"]["solve_s"])
 
+    cases = None
+    if not args.no_solve and not args.no_load_cases:
+        cases = load_cases()
+
     serving = None
     if args.serve_jobs > 0:
         serving = serve(args.serve_jobs, args.serve_workers)
@@ -232,6 +283,8 @@ def main():
                      "kernel_ms": ms_kernel},
         "matvec_fp32": {"kernel_ms": ms32, "GBps": nbytes32 / (ms32 * 1e-3) / 1e9},
         "solve": solves,
+        "shard": shard,
+        "load_cases": cases,
         "serving": serving,
         "cpu_baseline": cpu,
         "device": dev["name"],

@@ -62,7 +62,8 @@ enum { HEXA_SEMANTICS_GPU = 0, HEXA_SEMANTICS_CPU = 1 };
 enum { HEXA_DIAG_EXACT = 0, HEXA_DIAG_X_ENTRY = 1 };
 /* method reported by a solve; names follow GPUSolver.jl:314,463 */
 enum { HEXA_METHOD_JACOBI_GPU = 0, HEXA_METHOD_GMG_GPU = 1,
-       HEXA_METHOD_FAILED_CORRUPT_INPUT = 2, HEXA_METHOD_JACOBI_CPU = 3 };
+       HEXA_METHOD_FAILED_CORRUPT_INPUT = 2, HEXA_METHOD_JACOBI_CPU = 3,
+       HEXA_METHOD_NONE = 4 /* zero load on the free DOFs (CPUSolver.jl:222-224) */ };
 
 typedef struct hexa_handle hexa_handle;
 

@@ -233,3 +233,36 @@ def test_mg_odd_dimensions_support_on_max_face(grid, levels, precision):
     assert reps[0]["mg_levels"] == (levels or 3)
     # the coarse correction works: well under half the Jacobi-PCG iterations
     assert reps[0]["iterations"] < 0.5 * jacobi_its, (reps[0]["iterations"], jacobi_its)
+
+
+@pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
+def test_multi_rhs_load_cases(precision):
+    """Several load cases on one handle (the 8-RHS benchmark case in small):
+    each equals the single-case solve bitwise and the direct solve.  Load case
+    1 of this seed sits on a held node, so its free load is zero: u = 0 at
+    once, as CPUSolver.jl:222-224."""
+    say("multi rhs", precision)
+    nx, ny, nz, nrhs = 12, 6, 6, 3
+    p = P.cantilever(nx, ny, nz, seed=4, smooth_passes=1, lo=0.3)
+    f = P.point_loads(nx, ny, nz, nrhs, seed=4)
+    fac = oracle.stiffness_factor(p.rho, 1.0, 1e-9, 3.0)
+    Ke = oracle.ke_hex8(1, 1, 1, 0.3)
+    kw = dict(tol=1e-10, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, shift_factor=1e-10,
+              max_iter=2000)
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, nrhs=nrhs) as s:
+        s.set_problem(p.fixed, f.ravel())
+        s.set_density(p.rho)
+        u, reps, st = s.solve(**kw)
+    assert st == hexa.OK and len(reps) == nrhs
+    for r in range(nrhs):
+        with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s1:
+            s1.set_problem(p.fixed, f[r])
+            s1.set_density(p.rho)
+            u1, reps1, st1 = s1.solve(**kw)
+        assert st1 == hexa.OK
+        assert np.array_equal(u[r], u1[0])
+        assert reps[r]["iterations"] == reps1[0]["iterations"]
+        ud, _ = direct_solve(nx, ny, nz, Ke, fac, p.fixed, f[r])
+        assert rel(u[r], ud) < 1e-8
+        assert abs(reps[r]["compliance"] - f[r] @ ud) <= 1e-8 * abs(f[r] @ ud)
+    assert reps[1]["method_name"] == "None" and reps[1]["iterations"] == 0 and not np.any(u[1])

@@ -445,14 +445,26 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     int iters = 0, attempt_used = 1;
     double shift_used = 0.0;
     if (cpu) {
-        // CPUSolver.jl:192-255: one attempt, x0 = 0, clamp, every-iteration check
+        // CPUSolver.jl:192-255: one attempt, x0 = 0, clamp, every-iteration check;
+        // a zero load on the free DOFs returns u = 0 at once (:222-224)
         const double s = prm->shift_factor * maxd;
-        AttemptResult ar;
-        HX_TRY(run_attempt(h, prm, use_mg, s, &ar, &matvecs));
-        success = ar.halted == PCG_CONVERGED;
-        final_rel = ar.final_rel;
-        iters = ar.iterations;
+        launch_dot2(st, nd, h->d_b, h->d_b, h->d_partials);
+        launch_final_sum(st, 1, h->d_partials, h->d_scal + 3);
+        HX_TRY(check_launch());
+        double bb = 0.0;
+        HX_CHECK(hipMemcpyAsync(&bb, h->d_scal + 3, sizeof(double), hipMemcpyDeviceToHost, st));
+        HX_CHECK(hipStreamSynchronize(st));
         shift_used = s;
+        if (bb == 0.0) {
+            success = true;
+            R.method = HEXA_METHOD_NONE;
+        } else {
+            AttemptResult ar;
+            HX_TRY(run_attempt(h, prm, use_mg, s, &ar, &matvecs));
+            success = ar.halted == PCG_CONVERGED;
+            final_rel = ar.final_rel;
+            iters = ar.iterations;
+        }
     } else {
         // GPUSolver.jl:332-444
         while (true) {

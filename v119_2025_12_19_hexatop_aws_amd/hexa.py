@@ -34,7 +34,7 @@ PRECOND_JACOBI, PRECOND_MG = 0, 1
 MATVEC_AUTO, MATVEC_DENSE, MATVEC_FACTORED = 0, 1, 2
 SEMANTICS_GPU, SEMANTICS_CPU = 0, 1
 DIAG_EXACT, DIAG_X_ENTRY = 0, 1
-METHOD_NAMES = {0: "Jacobi_GPU", 1: "GMG_GPU", 2: "Failed_Corrupt_Input", 3: "Jacobi_CPU"}
+METHOD_NAMES = {0: "Jacobi_GPU", 1: "GMG_GPU", 2: "Failed_Corrupt_Input", 3: "Jacobi_CPU", 4: "None"}
 
 
 class HexaError(RuntimeError):

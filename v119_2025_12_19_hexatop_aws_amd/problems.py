@@ -185,7 +185,8 @@ def point_loads(nx, ny, nz, nrhs, seed=4) -> np.ndarray:
 
 
 def cantilever_c4(nrhs=8) -> Problem:
-    p = cantilever(256, 128, 128, seed=4)
+    # density as the 512x256x256 case: box-smoothed U(0,1) mapped to [0.001, 1]
+    p = cantilever(256, 128, 128, seed=4, smooth_passes=3)
     p.f = point_loads(256, 128, 128, nrhs, seed=4)
     p.meta.update(name="cantilever 256x128x128 x8 load cases")
     return p
