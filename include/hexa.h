@@ -56,7 +56,10 @@ extern "C" {
 enum { HEXA_PREC_FP64 = 0, HEXA_PREC_MIXED = 1 };            /* V-cycle precision */
 enum { HEXA_KE_FP64 = 0, HEXA_KE_F32_WIDENED = 1 };           /* element matrix    */
 enum { HEXA_HELM_BUG_COMPAT = 0, HEXA_HELM_CORRECTED = 1 };    /* filter element    */
-enum { HEXA_PRECOND_JACOBI = 0, HEXA_PRECOND_MG = 1 };
+/* HEXA_PRECOND_MG: geometric multigrid V-cycle (this library's);
+ * HEXA_PRECOND_REF_MG: the reference's additive two-level preconditioner
+ * z = D^-1 r + P (D_c + 1e-9)^-1 P^T r (GPUGeometricMultigrid.jl:253-346). */
+enum { HEXA_PRECOND_JACOBI = 0, HEXA_PRECOND_MG = 1, HEXA_PRECOND_REF_MG = 2 };
 enum { HEXA_MATVEC_AUTO = 0, HEXA_MATVEC_DENSE = 1, HEXA_MATVEC_FACTORED = 2 };
 enum { HEXA_SEMANTICS_GPU = 0, HEXA_SEMANTICS_CPU = 1 };
 enum { HEXA_DIAG_EXACT = 0, HEXA_DIAG_X_ENTRY = 1 };

@@ -419,6 +419,11 @@ int or_pcg_cpu(int nx, int ny, int nz, const double* Ke, const double* factor,
     return it;
 }
 
+static void refmg_setup(int nx, int ny, int nz, const double* kd8, const uint8_t* present,
+                        int* ncx, int* ncy, int* ncz, double** dc_out);
+static void refmg_apply(int nx, int ny, int nz, int cx, int cy, int cz, const double* dc,
+                        const uint8_t* fixed, const double* minv, const double* r, double* z);
+
 /* Jacobi-PCG with the GPU driver's control flow, in FP64.
  * R/src/Solvers/GPUSolver.jl:182-465 (gpu_matrix_free_cg_solve), Jacobi
  * branch: shift = cur_shift*max(diag_free) ADDED to the diagonal (:343,353);
@@ -429,11 +434,13 @@ int or_pcg_cpu(int nx, int ny, int nz, const double* Ke, const double* factor,
  * diagonal -> "Failed_Corrupt_Input" (:310-315).  diag_mode as or_diag.
  * Returns status: 0 converged, 1 not converged (best x returned),
  * 2 corrupt input.  info[0]=rel_res, info[1]=last iteration,
- * info[2]=attempt used, info[3]=shift used. */
+ * info[2]=attempt used, info[3]=shift used, info[4]=1 if the reference
+ * 2-level preconditioner fell back to Jacobi. */
 int or_pcg_gpu_semantics(int nx, int ny, int nz, const double* Ke, const double* factor,
                          const uint8_t* fixed, const double* f, const double* u_guess,
                          double tol, int max_iter, double shift_factor, int max_attempts,
                          double shift_multiplier, double stagnation_tol, int diag_mode,
+                         const double* refmg_kd8, const uint8_t* present,
                          double* x, double* info, int nchunks) {
     int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
     double* diag = (double*)malloc(sizeof(double) * ndof);
@@ -463,18 +470,29 @@ int or_pcg_gpu_semantics(int nx, int ny, int nz, const double* Ke, const double*
     double nb = sqrt(dotm(ndof, b, b));
     double cur = shift_factor, final_rel = 0.0;
     int success = 0;
+    /* the reference 2-level preconditioner when refmg_kd8 is given; it falls
+     * back to Jacobi on NaN/divergence (:396-399,434-437) */
+    const int want_mg = refmg_kd8 != NULL;
+    int use_mg = want_mg, cx = 0, cy = 0, cz = 0;
+    double* dc = NULL;
+    if (want_mg) refmg_setup(nx, ny, nz, refmg_kd8, present, &cx, &cy, &cz, &dc);
+    info[4] = 0.0;
+restart:
+    cur = shift_factor;
     for (int attempt = 1; attempt <= max_attempts; ++attempt) {
         double s = cur * maxd;
         int diverged = 0, ok = 0;
         info[2] = attempt; info[3] = s;
-        if (u_guess && attempt == 1) {
+        if (u_guess && attempt == 1 && use_mg == want_mg) {
             for (int64_t d = 0; d < ndof; ++d) x[d] = fixed[d] ? 0.0 : u_guess[d];
-        } else if (attempt > 1) {
+        } else if (attempt > 1 || use_mg != want_mg) {
             memset(x, 0, sizeof(double) * ndof);
         }
         for (int64_t d = 0; d < ndof; ++d) minv[d] = fixed[d] ? 0.0 : 1.0 / (diag[d] + s);
         apply_A(nx, ny, nz, Ke, factor, fixed, s, x, Ap, nchunks);
-        for (int64_t d = 0; d < ndof; ++d) { r[d] = b[d] - Ap[d]; z[d] = r[d] * minv[d]; p[d] = z[d]; }
+        for (int64_t d = 0; d < ndof; ++d) { r[d] = b[d] - Ap[d]; z[d] = r[d] * minv[d]; }
+        if (use_mg) refmg_apply(nx, ny, nz, cx, cy, cz, dc, fixed, minv, r, z);
+        for (int64_t d = 0; d < ndof; ++d) p[d] = z[d];
         double rz = dotm(ndof, r, z), best_rel = INFINITY;
         for (int k = 1; k <= max_iter; ++k) {
             apply_A(nx, ny, nz, Ke, factor, fixed, s, p, Ap, nchunks);
@@ -492,8 +510,12 @@ int or_pcg_gpu_semantics(int nx, int ny, int nz, const double* Ke, const double*
                 if (isnan(rel) || isinf(rel)) { diverged = 1; break; }
                 if (rel > 10.0 || (k > 500 && rel > best_rel * 100.0)) { diverged = 1; break; }
             }
+            if (use_mg) {
+                refmg_apply(nx, ny, nz, cx, cy, cz, dc, fixed, minv, r, z);
+            } else {
 #pragma omp parallel for schedule(static)
-            for (int64_t d = 0; d < ndof; ++d) z[d] = r[d] * minv[d];
+                for (int64_t d = 0; d < ndof; ++d) z[d] = r[d] * minv[d];
+            }
             double rz_new = dotm(ndof, r, z);
             double beta = rz_new / rz;
 #pragma omp parallel for schedule(static)
@@ -501,6 +523,7 @@ int or_pcg_gpu_semantics(int nx, int ny, int nz, const double* Ke, const double*
             rz = rz_new;
         }
         if (ok) { success = 1; break; }
+        if (diverged && use_mg) { use_mg = 0; info[4] = 1.0; goto restart; }
         if (diverged && attempt != max_attempts) { cur *= shift_multiplier; continue; }
         if (stagnation_tol > 0.0 && best_rel < stagnation_tol) {
             success = 1;
@@ -512,9 +535,93 @@ int or_pcg_gpu_semantics(int nx, int ny, int nz, const double* Ke, const double*
     if (!success) memcpy(x, best, sizeof(double) * ndof);
     info[0] = final_rel;
     status = success ? 0 : 1;
+    free(dc);
 done:
     free(diag); free(minv); free(r); free(z); free(p); free(Ap); free(best); free(b);
     return status;
+}
+
+/* ---------------------------------------- reference 2-level preconditioner */
+/* R/src/Solvers/GPUGeometricMultigrid.jl:253-315 (set-up) and :317-346
+ * (apply): coarse grid of floor(n/2) (at least 1) bricks of twice the size;
+ * fine node coordinates divided by the coarse size (here exactly i/2, j/2,
+ * k/2); coarse element density 1 where a present fine node falls in it
+ * (floor of the normalised coordinates, :209-219), 0.001 elsewhere; coarse
+ * diagonal D_c = 1 + sum over coarse elements of rho_c * kd (kd: the x-entry
+ * diagonal of the coarse brick's Ke, used for all three components, :41-54);
+ * apply: z = minv r, r_c = P^T r by trilinear weights from the normalised
+ * coordinates (:57-126), x_c = r_c / (D_c + 1e-9), z += P x_c on the free
+ * DOFs (:128-190, 339-343).  kd8: the 8 corner x-diagonal entries (corner
+ * order as the element's node order); present: per fine node, or NULL for
+ * all present. */
+static void refmg_setup(int nx, int ny, int nz, const double* kd8, const uint8_t* present,
+                        int* ncx, int* ncy, int* ncz, double** dc_out) {
+    const int cx = nx / 2 > 1 ? nx / 2 : 1, cy = ny / 2 > 1 ? ny / 2 : 1, cz = nz / 2 > 1 ? nz / 2 : 1;
+    const int64_t nNc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1), nEc = (int64_t)cx * cy * cz;
+    double* rho_c = (double*)malloc(sizeof(double) * nEc);
+    for (int64_t e = 0; e < nEc; ++e) rho_c[e] = 0.001;
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const int64_t n = i + (int64_t)(nx + 1) * (j + (int64_t)(ny + 1) * k);
+                if (present && !present[n]) continue;
+                const int ix = (int)floor(0.5 * i), iy = (int)floor(0.5 * j), iz = (int)floor(0.5 * k);
+                if (ix < cx && iy < cy && iz < cz) rho_c[ix + (int64_t)cx * (iy + (int64_t)cy * iz)] = 1.0;
+            }
+    double* dc = (double*)malloc(sizeof(double) * 3 * nNc);
+    for (int64_t d = 0; d < 3 * nNc; ++d) dc[d] = 1.0;
+    int64_t nd[8];
+    for (int64_t e = 0; e < nEc; ++e) {
+        elem_nodes(e, cx, cy, nd);
+        for (int a = 0; a < 8; ++a)
+            for (int c = 0; c < 3; ++c) dc[3 * nd[a] + c] += kd8[a] * rho_c[e];
+    }
+    free(rho_c);
+    *ncx = cx; *ncy = cy; *ncz = cz;
+    *dc_out = dc;
+}
+
+static void refmg_apply(int nx, int ny, int nz, int cx, int cy, int cz, const double* dc,
+                        const uint8_t* fixed, const double* minv, const double* r, double* z) {
+    const int64_t nN = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    const int64_t nNc = (int64_t)(cx + 1) * (cy + 1) * (cz + 1);
+    double* rc = (double*)calloc((size_t)(3 * nNc), sizeof(double));
+    for (int64_t d = 0; d < 3 * nN; ++d) z[d] = fixed[d] ? 0.0 : r[d] * minv[d];
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1)
+            for (int64_t d = 0; d < 3 * nNc; ++d) rc[d] = rc[d] / (dc[d] + 1e-9);
+        for (int64_t n = 0; n < nN; ++n) {
+            const int i = (int)(n % (nx + 1)), j = (int)((n / (nx + 1)) % (ny + 1)), k = (int)(n / ((int64_t)(nx + 1) * (ny + 1)));
+            const double x = 0.5 * i, y = 0.5 * j, zz = 0.5 * k;
+            const int i0 = (int)floor(x), j0 = (int)floor(y), k0 = (int)floor(zz);
+            const double tx = x - i0, ty = y - j0, tz = zz - k0;
+            double acc[3] = {0, 0, 0};
+            for (int q = 0; q < 8; ++q) {
+                const int a = q & 1, b = (q >> 1) & 1, c = (q >> 2) & 1;
+                const int I = i0 + a, J = j0 + b, K = k0 + c;
+                if (I > cx || J > cy || K > cz) continue;
+                const double w = (a ? tx : 1 - tx) * (b ? ty : 1 - ty) * (c ? tz : 1 - tz);
+                const int64_t m = I + (int64_t)(cx + 1) * (J + (int64_t)(cy + 1) * K);
+                for (int comp = 0; comp < 3; ++comp) {
+                    if (pass == 0) rc[3 * m + comp] += w * (fixed[3 * n + comp] ? 0.0 : r[3 * n + comp]);
+                    else acc[comp] += w * rc[3 * m + comp];
+                }
+            }
+            if (pass == 1)
+                for (int comp = 0; comp < 3; ++comp)
+                    if (!fixed[3 * n + comp]) z[3 * n + comp] += acc[comp];
+        }
+    }
+    free(rc);
+}
+
+void or_refmg_apply(int nx, int ny, int nz, const double* kd8, const uint8_t* present,
+                    const uint8_t* fixed, const double* minv, const double* r, double* z) {
+    int cx, cy, cz;
+    double* dc;
+    refmg_setup(nx, ny, nz, kd8, present, &cx, &cy, &cz, &dc);
+    refmg_apply(nx, ny, nz, cx, cy, cz, dc, fixed, minv, r, z);
+    free(dc);
 }
 
 /* ------------------------------------------------------- Helmholtz filter */

@@ -65,7 +65,9 @@ def lib():
         L.or_pcg_gpu_semantics.restype = C.c_int
         L.or_pcg_gpu_semantics.argtypes = [C.c_int] * 3 + [
             _dp, _dp, _u8, _dp, C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
-            C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int]
+            C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int]
+        L.or_refmg_apply.restype = None
+        L.or_refmg_apply.argtypes = [C.c_int] * 3 + [_dp, C.c_void_p, _u8, _dp, _dp, _dp]
         L.or_filter.restype = C.c_int
         L.or_filter.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [
             C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
@@ -144,20 +146,43 @@ def pcg_cpu(grid, Ke, factor, fixed, f, tol=1e-8, max_iter=10000, shift_factor=0
     return x, rel.value, it, sh.value
 
 
+def refmg_kd8(dx=1.0, dy=1.0, dz=1.0):
+    """x-entry diagonal of the coarse brick's Ke (twice the fine size, E = 1,
+    nu = 0.3, Float32 evaluation) at the 8 corners
+    (R/src/Solvers/GPUGeometricMultigrid.jl:299-302, Element.jl:104-111)."""
+    Ke = ke_hex8(2.0 * dx, 2.0 * dy, 2.0 * dz, 0.3, f32=True)
+    return np.array([Ke[3 * a, 3 * a] for a in range(8)])
+
+
+def refmg_apply(grid, kd8, fixed, minv, r, present=None):
+    """The reference's additive 2-level preconditioner z = M r
+    (GPUGeometricMultigrid.jl:253-346)."""
+    nx, ny, nz = grid
+    z = np.zeros(ndof(nx, ny, nz))
+    pr = None if present is None else np.ascontiguousarray(present, dtype=np.uint8)
+    lib().or_refmg_apply(nx, ny, nz, _f64(kd8), _ptr(pr), np.ascontiguousarray(fixed, dtype=np.uint8),
+                         _f64(minv), _f64(r), z)
+    return z
+
+
 def pcg_gpu_semantics(grid, Ke, factor, fixed, f, u_guess=None, tol=1e-8, max_iter=10000,
                       shift_factor=1e-10, max_attempts=3, shift_multiplier=10.0,
-                      stagnation_tol=0.0, diag_mode=1, nchunks=0):
-    """GPUSolver.gpu_matrix_free_cg_solve control flow, Jacobi branch, FP64."""
+                      stagnation_tol=0.0, diag_mode=1, refmg_kd8=None, present=None, nchunks=0):
+    """GPUSolver.gpu_matrix_free_cg_solve control flow, FP64: Jacobi, or the
+    reference 2-level preconditioner (with its Jacobi fallback) when
+    ``refmg_kd8`` is given."""
     nx, ny, nz = grid
     x = np.zeros(ndof(nx, ny, nz))
-    info = np.zeros(4)
+    info = np.zeros(5)
     g = None if u_guess is None else _f64(u_guess)
+    kd = None if refmg_kd8 is None else _f64(refmg_kd8)
+    pr = None if present is None else np.ascontiguousarray(present, dtype=np.uint8)
     st = lib().or_pcg_gpu_semantics(nx, ny, nz, _f64(Ke).ravel(), _f64(factor),
                                     np.ascontiguousarray(fixed, dtype=np.uint8), _f64(f), _ptr(g),
                                     tol, max_iter, shift_factor, max_attempts, shift_multiplier,
-                                    stagnation_tol, diag_mode, x, info, nchunks)
+                                    stagnation_tol, diag_mode, _ptr(kd), _ptr(pr), x, info, nchunks)
     return x, st, dict(rel_res=info[0], iterations=int(info[1]), attempt=int(info[2]),
-                       shift=info[3])
+                       shift=info[3], mg_fallback=bool(info[4]))
 
 
 def filter_density(grid, h, radius, rho, bug_compat=True, tol=1e-6, max_iter=200, check_every=10):

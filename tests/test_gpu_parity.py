@@ -266,3 +266,50 @@ def test_multi_rhs_load_cases(precision):
         assert rel(u[r], ud) < 1e-8
         assert abs(reps[r]["compliance"] - f[r] @ ud) <= 1e-8 * abs(f[r] @ ud)
     assert reps[1]["method_name"] == "None" and reps[1]["iterations"] == 0 and not np.any(u[1])
+
+
+@pytest.mark.parametrize("grid", [(8, 4, 4), (7, 5, 3)])
+def test_reference_two_level_preconditioner_matches_oracle(grid):
+    """HEXA_PRECOND_REF_MG (GPUGeometricMultigrid.jl:253-346) applied on the
+    GPU equals the oracle's restatement: floor(n/2) coarse grid, x-entry
+    coarse diagonal with unit coarse density, z = D^-1 r + P (D_c+1e-9)^-1 P^T r."""
+    say("refmg apply", grid)
+    nx, ny, nz = grid
+    fixed, f, rho = cantilever_arrays(nx, ny, nz, 12, lo=0.2)
+    Ke = oracle.ke_hex8(1, 1, 1, 0.3)
+    fac = oracle.stiffness_factor(rho, 1.0, 1e-9, 3.0)
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, diag_mode=hexa.DIAG_X_ENTRY) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        u, reps, st = s.solve(tol=1e-3, precond=hexa.PRECOND_REF_MG)
+        assert reps[0]["method_name"] == "GMG_GPU" and reps[0]["mg_levels"] == 2
+        rng = np.random.default_rng(8)
+        r = rng.standard_normal(fixed.size) * (fixed == 0)
+        z = s.apply_precond(r)
+        shift = reps[0]["shift_used"]
+    d = oracle.diag(grid, Ke, fac, mode=1)
+    minv = np.where(fixed == 0, 1.0 / (d + shift), 0.0)
+    zo = oracle.refmg_apply(grid, oracle.refmg_kd8(), fixed, minv, r)
+    assert np.abs(z - zo).max() <= 1e-12 * np.abs(zo).max()
+
+
+def test_reference_two_level_solve_matches_oracle_control_flow():
+    """GPU-semantics PCG with the reference two-level preconditioner: same
+    status, attempt, shift and check-point iteration as the oracle, and the
+    golden direct solution."""
+    say("refmg solve")
+    grid, fixed, f, rho, ug = _golden_problem("b")
+    Ke = oracle.ke_hex8(1, 1, 1, 0.3)
+    fac = oracle.stiffness_factor(rho, 1.0, 1e-9, 3.0)
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, diag_mode=hexa.DIAG_X_ENTRY) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        u, reps, st = s.solve(tol=1e-8, max_iter=20000, precond=hexa.PRECOND_REF_MG, shift_factor=1e-10)
+    uo, sto, info = oracle.pcg_gpu_semantics(grid, Ke, fac, fixed, f, tol=1e-8, max_iter=20000,
+                                             shift_factor=1e-10, refmg_kd8=oracle.refmg_kd8())
+    r = reps[0]
+    assert st == hexa.OK and sto == 0 and not info["mg_fallback"]
+    assert r["method_name"] == "GMG_GPU" and r["attempt"] == info["attempt"]
+    assert r["iterations"] == info["iterations"]
+    assert rel(u[0], uo) < 1e-6
+    assert rel(u[0], ug) < 1e-6

@@ -261,3 +261,60 @@ def test_oc_update_invariants():
     # top88's loose stop still lands close to the volume target
     xl, _, s2 = oracle.oc_update(x, dc, dv, w, 0.3, 0.2, bisect_tol=1e-3)
     assert s2 < steps and abs(w @ xl - 0.3 * n) < 2e-3 * n
+
+
+def _interp_1d(nf, nc):
+    """Fine node i <- coarse node I with weight max(0, 1 - |i/2 - I|)."""
+    P = np.zeros((nf + 1, nc + 1))
+    for i in range(nf + 1):
+        for I in range(nc + 1):
+            P[i, I] = max(0.0, 1.0 - abs(0.5 * i - I))
+    return P
+
+
+@pytest.mark.parametrize("grid,prune", [((4, 3, 2), False), ((5, 3, 1), False), ((6, 4, 4), True)])
+def test_oracle_reference_two_level_preconditioner(grid, prune):
+    """The reference's additive 2-level preconditioner (GPUGeometricMultigrid.jl)
+    against an explicit Kronecker-product prolongation and an assembled coarse
+    diagonal: z = minv r + P (D_c + 1e-9)^-1 P^T r on the free DOFs."""
+    nx, ny, nz = grid
+    cx, cy, cz = max(1, nx // 2), max(1, ny // 2), max(1, nz // 2)
+    Pk = np.kron(_interp_1d(nz, cz), np.kron(_interp_1d(ny, cy), _interp_1d(nx, cx)))  # nodes x coarse nodes
+    nN, nNc = Pk.shape
+    rng = np.random.default_rng(5)
+    present = np.ones(nN, np.uint8)
+    if prune:   # drop the fine nodes of coarse element (1,0,1): i in {2,3}, j in {0,1}, k in {2,3}
+        for k in (2, 3):
+            for j in (0, 1):
+                for i in (2, 3):
+                    present[i + (nx + 1) * (j + (ny + 1) * k)] = 0
+    fixed = (rng.uniform(size=3 * nN) < 0.2).astype(np.uint8)
+    fixed[np.repeat(present == 0, 3)] = 1
+    minv = np.where(fixed == 1, 0.0, rng.uniform(0.5, 2.0, 3 * nN))
+    r = rng.standard_normal(3 * nN)
+    kd8 = oracle.refmg_kd8(1.0, 1.0, 1.0)
+    # coarse density: 1 where a present fine node has floor(i/2) = I (per axis) inside the coarse box
+    rho_c = np.full(cx * cy * cz, 0.001)
+    for n in np.flatnonzero(present):
+        i, j, k = n % (nx + 1), (n // (nx + 1)) % (ny + 1), n // ((nx + 1) * (ny + 1))
+        if i // 2 < cx and j // 2 < cy and k // 2 < cz:
+            rho_c[i // 2 + cx * (j // 2 + cy * (k // 2))] = 1.0
+    Dc = np.ones(nNc)
+    for e in range(cx * cy * cz):
+        I, J, K = e % cx, (e // cx) % cy, e // (cx * cy)
+        for a, (di, dj, dk) in enumerate([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                                          (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]):
+            Dc[(I + di) + (cx + 1) * ((J + dj) + (cy + 1) * (K + dk))] += kd8[a] * rho_c[e]
+    free = fixed == 0
+    rf = np.where(free, r, 0.0).reshape(nN, 3)
+    xc = (Pk.T @ rf) / (Dc[:, None] + 1e-9)
+    z_exp = np.where(free, minv * r + (Pk @ xc).reshape(-1), 0.0)
+    z = oracle.refmg_apply(grid, kd8, fixed, minv, r, present=present if prune else None)
+    assert np.abs(z - z_exp).max() <= 1e-13 * np.abs(z_exp).max()
+    # the prolongation reproduces linear fields on the nodes it covers fully
+    lin_c = np.array([I for K in range(cz + 1) for J in range(cy + 1) for I in range(cx + 1)], float)
+    lin_f = Pk @ lin_c
+    for n in range(nN):
+        i = n % (nx + 1)
+        if i <= 2 * cx:
+            assert abs(lin_f[n] - 0.5 * i) < 1e-14 or (n // (nx + 1)) % (ny + 1) > 2 * cy or n // ((nx + 1) * (ny + 1)) > 2 * cz

@@ -283,6 +283,40 @@ void apply_mg(hexa_handle* h, const double* r, double* z) {
 }
 
 
+// Reference two-level preconditioner set-up (GPUGeometricMultigrid.jl:253-315):
+// coarse grid of floor(n/2) (at least 1) bricks of twice the size, the
+// inverse coarse diagonal from the x-entry diagonal of the coarse brick's
+// Float32 Ke with E = 1 and nu = 0.3 (as :299-302), an all-free coarse mask.
+int32_t refmg_build(hexa_handle* h) {
+    if (h->refmg_ready) return HEXA_OK;
+    const Grid3& f = h->g;
+    h->rg = make_grid(std::max(1, f.nx / 2), std::max(1, f.ny / 2), std::max(1, f.nz / 2));
+    const size_t ndc = (size_t)(3 * h->rg.nn);
+    HX_TRY(dalloc(h, &h->d_rc, ndc));
+    HX_TRY(dalloc(h, &h->d_xc, ndc));
+    HX_TRY(dalloc(h, &h->d_dcinv, ndc));
+    HX_TRY(dalloc_bytes(h, (void**)&h->d_rcmask, (size_t)h->rg.nn));
+    double Kc[576], kd[8];
+    element_stiffness(2.0 * h->grid.dx, 2.0 * h->grid.dy, 2.0 * h->grid.dz, 0.3, HEXA_KE_F32_WIDENED, Kc);
+    for (int a = 0; a < 8; ++a) kd[a] = Kc[(3 * a) * 24 + 3 * a];
+    HX_CHECK(hipMemsetAsync(h->d_rcmask, 0, (size_t)h->rg.nn, h->stream));
+    launch_refmg_dcinv(h->stream, f, h->rg, h->d_mask, kd, h->d_dcinv);
+    HX_TRY(check_launch());
+    h->refmg_ready = true;
+    return HEXA_OK;
+}
+
+// z = minv r + P (D_c + 1e-9)^-1 P^T r on the free DOFs
+// (GPUGeometricMultigrid.jl:317-346).
+void apply_refmg(hexa_handle* h, const double* r, double* z) {
+    hipStream_t s = h->stream;
+    const int64_t nd = 3 * h->g.nn;
+    launch_scale_dinv<double, double>(s, nd, r, h->d_minv, 1.0, z);
+    launch_restrict<double>(s, h->g, h->rg, r, h->d_rcmask, h->d_rc);
+    launch_scale_dinv<double, double>(s, 3 * h->rg.nn, h->d_rc, h->d_dcinv, 1.0, h->d_xc);
+    launch_prolong_add<double>(s, h->g, h->rg, h->d_xc, h->d_mask, z);
+}
+
 // b = f masked to the free DOFs.
 __global__ void k_mask_copy(int64_t nd, const double* f, const uint8_t* __restrict__ mask, double* b) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nd;
@@ -304,8 +338,10 @@ struct AttemptResult {
 
 // One PCG attempt with shift s from the current d_x.  Returns the final
 // device state (synchronised).
-int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, double s,
+// pk: 0 Jacobi, 1 multigrid V-cycle, 2 reference two-level preconditioner.
+int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double s,
                     AttemptResult* res, int64_t* matvecs) {
+    const bool use_mg = pk == 1;
     hipStream_t st = h->stream;
     const int64_t nd = 3 * h->g.nn;
     const bool cpu = prm->semantics == HEXA_SEMANTICS_CPU;
@@ -321,7 +357,8 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
     elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_RESID, h->d_fac, h->d_mask, h->d_x, h->d_b,
                             nullptr, 0.0, s, h->d_r);
     ++*matvecs;
-    if (use_mg) apply_mg(h, h->d_r, h->d_z);
+    if (pk == 1) apply_mg(h, h->d_r, h->d_z);
+    else if (pk == 2) apply_refmg(h, h->d_r, h->d_z);
     else launch_scale_dinv<double, double>(st, nd, h->d_r, h->d_minv, 1.0, h->d_z);
     HX_CHECK(hipMemcpyAsync(h->d_p, h->d_z, nd * sizeof(double), hipMemcpyDeviceToDevice, st));
     launch_pcg_rz(st, nd, h->d_r, h->d_z, h->d_partials);
@@ -346,9 +383,14 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
             launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
             launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
         }
-        launch_pcg_update(st, nd, h->d_state, h->d_x, h->d_p, h->d_r, h->d_Ap, use_mg ? nullptr : h->d_minv,
+        launch_pcg_update(st, nd, h->d_state, h->d_x, h->d_p, h->d_r, h->d_Ap, pk == 0 ? h->d_minv : nullptr,
                           h->d_partials, r32);
         const int check = cpu ? 1 : ((k % check_every == 0) || k == 1);
+        if (pk == 2) {
+            // z = M r and (r, z) for the standard beta (GPUSolver.jl:409-417)
+            apply_refmg(h, h->d_r, h->d_z);
+            launch_pcg_rz(st, nd, h->d_r, h->d_z, h->d_partials + RED_BLOCKS);
+        }
         launch_pcg_check(st, h->d_state, h->d_partials, k, check, cpu ? 1 : 0, use_mg ? 0 : 1);
         if (check && !cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
         if (mx) {
@@ -360,6 +402,8 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, bool use_mg, d
             apply_mg(h, h->d_r, h->d_z);
             launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);
             launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr);
+        } else if (pk == 2) {
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr);
         } else {
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, nullptr, h->d_r, h->d_minv);
@@ -408,9 +452,11 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
         return HEXA_E_NAN_INPUT;
     }
 
-    const bool want_mg = prm->precond == HEXA_PRECOND_MG;
+    // multigrid kind: 1 V-cycle, 2 reference two-level preconditioner
+    const int mg_kind = prm->precond == HEXA_PRECOND_MG ? 1 : (prm->precond == HEXA_PRECOND_REF_MG ? 2 : 0);
+    const bool want_mg = mg_kind != 0;
     bool use_mg = want_mg;
-    if (use_mg && (h->precond_ready != HEXA_PRECOND_MG)) {
+    if (mg_kind == 1 && (h->precond_ready != HEXA_PRECOND_MG)) {
         int32_t s = mg_build(h);
         if (s != HEXA_OK) {
             if (s == HEXA_E_OOM) return s;
@@ -419,9 +465,15 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
             h->precond_ready = HEXA_PRECOND_MG;
             h->mg_shift = -1.0;
         }
+    } else if (mg_kind == 2) {
+        int32_t s = refmg_build(h);
+        if (s != HEXA_OK) {
+            if (s == HEXA_E_OOM) return s;
+            use_mg = false;
+        }
     }
     // the first attempt's smoother and coarse inverse belong to the set-up
-    if (use_mg) HX_TRY(mg_set_shift(h, prm->shift_factor * maxd));
+    if (use_mg && mg_kind == 1) HX_TRY(mg_set_shift(h, prm->shift_factor * maxd));
     // warm start: from the host buffer, or (u_inout == NULL) from the
     // device-resident solution of the previous solve on this handle
     const bool warm = prm->warm_start && (u_inout || h->have_solution);
@@ -460,7 +512,7 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
             R.method = HEXA_METHOD_NONE;
         } else {
             AttemptResult ar;
-            HX_TRY(run_attempt(h, prm, use_mg, s, &ar, &matvecs));
+            HX_TRY(run_attempt(h, prm, use_mg ? mg_kind : 0, s, &ar, &matvecs));
             success = ar.halted == PCG_CONVERGED;
             final_rel = ar.final_rel;
             iters = ar.iterations;
@@ -482,7 +534,7 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
                     launch_fill<double>(st, nd, 0.0, h->d_x);
                 }
                 AttemptResult ar;
-                HX_TRY(run_attempt(h, prm, use_mg, s, &ar, &matvecs));
+                HX_TRY(run_attempt(h, prm, use_mg ? mg_kind : 0, s, &ar, &matvecs));
                 iters = ar.iterations;
                 final_rel = ar.final_rel;
                 best_rel = ar.best_rel;
@@ -530,7 +582,8 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     R.shift_used = shift_used;
     R.converged = success ? 1 : 0;
     R.matvecs = matvecs;
-    R.mg_levels = use_mg ? (int32_t)h->levels.size() : 0;
+    R.mg_levels = use_mg ? (mg_kind == 1 ? (int32_t)h->levels.size() : 2) : 0;
+    h->last_mg = use_mg ? mg_kind : 0;
     if (rep) *rep = R;
     return success ? HEXA_OK : HEXA_E_NOT_CONVERGED;
 }
@@ -721,6 +774,7 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
     HX_CHECK(hipStreamSynchronize(st));
     h->have_problem = true;
     h->precond_ready = -1;
+    h->refmg_ready = false;   // the coarse diagonal depends on the excluded nodes
     h->have_density = false;  // the factor depends on the active set
     h->have_solution = false;
     return HEXA_OK;
@@ -814,11 +868,16 @@ int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t prec
 
 int32_t hexa_apply_precond(hexa_handle* h, const double* r, double* z) {
     if (!h || !r || !z) { set_error("null argument"); return HEXA_E_INVALID; }
-    if (h->precond_ready != HEXA_PRECOND_MG) { set_error("no multigrid set-up: solve with HEXA_PRECOND_MG first"); return HEXA_E_INVALID; }
+    const bool ref = h->last_mg == 2 && h->refmg_ready;
+    if (!ref && h->precond_ready != HEXA_PRECOND_MG) {
+        set_error("no multigrid set-up: solve with HEXA_PRECOND_MG or HEXA_PRECOND_REF_MG first");
+        return HEXA_E_INVALID;
+    }
     const int64_t nd = 3 * h->g.nn;
     hipStream_t st = h->stream;
     HX_CHECK(hipMemcpyAsync(h->d_r, r, sizeof(double) * nd, hipMemcpyHostToDevice, st));
-    apply_mg(h, h->d_r, h->d_z);
+    if (ref) apply_refmg(h, h->d_r, h->d_z);
+    else apply_mg(h, h->d_r, h->d_z);
     HX_TRY(check_launch());
     HX_CHECK(hipMemcpyAsync(z, h->d_z, sizeof(double) * nd, hipMemcpyDeviceToHost, st));
     HX_CHECK(hipStreamSynchronize(st));

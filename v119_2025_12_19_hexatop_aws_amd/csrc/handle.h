@@ -77,6 +77,15 @@ struct hexa_handle {
     double *d_gj_row = nullptr, *d_gj_col = nullptr;
     int64_t n_coarse = 0;
     double mg_shift = -1.0;         // shift the smoother/coarse inverse were built for
+    int last_mg = -1;               // multigrid kind the last solve applied
+    // reference two-level preconditioner: floor(n/2) coarse grid, coarse
+    // residual/correction and inverse coarse diagonal, an all-free coarse mask
+    bool refmg_ready = false;
+    hexa::Grid3 rg{};
+    double* d_rc = nullptr;
+    double* d_xc = nullptr;
+    double* d_dcinv = nullptr;
+    uint8_t* d_rcmask = nullptr;
 
     // Helmholtz filter and optimisation-step work arrays
     double* d_Ae = nullptr;         // 64

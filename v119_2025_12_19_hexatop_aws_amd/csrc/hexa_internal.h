@@ -132,6 +132,8 @@ void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, cons
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf);
+void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
+                        const double* kd8, double* dcinv);
 void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,
                             const uint8_t* mask, double shift, double* A);
 void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf);

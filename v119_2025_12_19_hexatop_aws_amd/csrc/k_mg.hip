@@ -76,6 +76,9 @@ __global__ void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
         for (int a = 0; a < nk; ++a)
             for (int bb = 0; bb < nj; ++bb)
                 for (int cc = 0; cc < ni; ++cc) {
+                    // a coarse grid of floor(n/2) elements does not cover the
+                    // last fine node of an odd axis: its far weight is dropped
+                    if (i0 + cc >= CNX || j0 + bb >= CNY || k0 + a >= c.nz + 1) continue;
                     const int64_t m = (i0 + cc) + (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
                     s0 += xc[3 * m];
                     s1 += xc[3 * m + 1];
@@ -85,6 +88,43 @@ __global__ void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
         if (!(mb & 1)) xf[3 * n] += w * s0;
         if (!(mb & 2)) xf[3 * n + 1] += w * s1;
         if (!(mb & 4)) xf[3 * n + 2] += w * s2;
+    }
+}
+
+// Reference 2-level preconditioner, coarse diagonal
+// (R/src/Solvers/GPUGeometricMultigrid.jl:253-315): per coarse node,
+// 1 + sum over its coarse elements of rho_c * kd[corner], with rho_c = 1 when
+// a present fine node (mask bit 3 clear) has floor(i/2, j/2, k/2) equal to
+// the element's index, else 0.001; stored as 1/(D + 1e-9) for all three
+// components (:334).  Node-centric: no atomics.
+struct Kd8 {
+    double v[8];
+};
+__device__ constexpr int MCX_H[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__device__ constexpr int MCY_H[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__device__ constexpr int MCZ_H[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+__global__ void k_refmg_dcinv(Grid3 f, Grid3 c, const uint8_t* __restrict__ fmask, Kd8 kd, double* __restrict__ dcinv) {
+    const int CNX = c.nx + 1, CNY = c.ny + 1, FNX = f.nx + 1, FNY = f.ny + 1;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < c.nn;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(n % CNX), J = (int)((n / CNX) % CNY), K = (int)(n / ((int64_t)CNX * CNY));
+        double d = 1.0;
+        for (int q = 0; q < 8; ++q) {
+            // element (I - cx, J - cy, K - cz) has this node as its corner q
+            const int eI = I - MCX_H[q], eJ = J - MCY_H[q], eK = K - MCZ_H[q];
+            if (eI < 0 || eJ < 0 || eK < 0 || eI >= c.nx || eJ >= c.ny || eK >= c.nz) continue;
+            bool present = false;
+            for (int t = 0; t < 8 && !present; ++t) {
+                const int i = 2 * eI + (t & 1), j = 2 * eJ + ((t >> 1) & 1), k = 2 * eK + (t >> 2);
+                if (i > f.nx || j > f.ny || k > f.nz) continue;
+                present = !(fmask[i + (int64_t)FNX * (j + (int64_t)FNY * k)] & 8);
+            }
+            d += kd.v[q] * (present ? 1.0 : 0.001);
+        }
+        const double inv = 1.0 / (d + 1e-9);
+        dcinv[3 * n] = inv;
+        dcinv[3 * n + 1] = inv;
+        dcinv[3 * n + 2] = inv;
     }
 }
 
@@ -185,6 +225,13 @@ template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf) {
     k_prolong_add<T><<<grid_for(fine.nn), 256, 0, s>>>(fine, coarse, xc, fmask, xf);
+}
+
+void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
+                        const double* kd8, double* dcinv) {
+    Kd8 kd;
+    for (int q = 0; q < 8; ++q) kd.v[q] = kd8[q];
+    k_refmg_dcinv<<<grid_for(coarse.nn), 256, 0, s>>>(fine, coarse, fmask, kd, dcinv);
 }
 
 void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,

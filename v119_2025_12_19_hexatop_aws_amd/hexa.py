@@ -30,7 +30,7 @@ OK, E_INVALID, E_OOM, E_NAN_INPUT, E_NOT_CONVERGED, E_DEVICE = 0, -1, -2, -3, -4
 PREC_FP64, PREC_MIXED = 0, 1
 KE_FP64, KE_F32_WIDENED = 0, 1
 HELM_BUG_COMPAT, HELM_CORRECTED = 0, 1
-PRECOND_JACOBI, PRECOND_MG = 0, 1
+PRECOND_JACOBI, PRECOND_MG, PRECOND_REF_MG = 0, 1, 2
 MATVEC_AUTO, MATVEC_DENSE, MATVEC_FACTORED = 0, 1, 2
 SEMANTICS_GPU, SEMANTICS_CPU = 0, 1
 DIAG_EXACT, DIAG_X_ENTRY = 0, 1
@@ -460,6 +460,9 @@ def solve_system_gpu(nodes, elements, E, nu, bc, f, density, *, max_iter=40000, 
     active = np.zeros(nE, dtype=np.uint8)
     active[eid] = 1
     want_mg = str(sp.get("preconditioner", "jacobi")) == "multigrid"
+    # "multigrid" is the reference's two-level preconditioner unless
+    # solver_parameters.mg_mode asks for this library's V-cycle
+    mg_precond = PRECOND_MG if str(sp.get("mg_mode", "reference")) == "vcycle" else PRECOND_REF_MG
     prec = PREC_MIXED if str(sp.get("precision", "fp64")) == "mixed" else PREC_FP64
     opts = dict(E0=float(E), Emin=0.0, penal=1.0, nu=float(nu), precision=prec,
                 ke_mode=KE_F32_WIDENED if str(sp.get("ke_mode", "f32_widened")) == "f32_widened" else KE_FP64,
@@ -479,7 +482,7 @@ def solve_system_gpu(nodes, elements, E, nu, bc, f, density, *, max_iter=40000, 
                               max_shift_attempts=int(sp.get("max_shift_attempts", 3)),
                               shift_multiplier=float(sp.get("shift_multiplier", 10.0)),
                               stagnation_tol=float(sp.get("stagnation_tolerance", 0.0)),
-                              precond=PRECOND_MG if want_mg else PRECOND_JACOBI, warm_start=int(warm))
+                              precond=mg_precond if want_mg else PRECOND_JACOBI, warm_start=int(warm))
     rep = reps[0]
     U = u[0].reshape(nN, 3)[nid].reshape(-1)
     if st == E_NAN_INPUT:
