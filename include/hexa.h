@@ -145,6 +145,17 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
 int32_t hexa_set_density(hexa_handle* h, const double* rho);
 int32_t hexa_set_density_dev(hexa_handle* h, const double* d_rho);
 
+/* Void pruning on the device, replacing the host-side
+ * MeshPruner.prune_system step of Solver.solve_system (Solver.jl:85-116,
+ * MeshPruner.jl:10-87): with the handle's current density, elements with
+ * Float32(rho) <= Float32(min_stiffness_threshold) * 1.01f become inactive
+ * when fewer than 99% of the elements would stay active (otherwise none
+ * is); nodes no active element touches are excluded.  The node numbering
+ * is unchanged, so a warm start needs no remap (Solver.jl:97-108).
+ * min_stiffness_threshold < 0 makes every element active again.
+ * n_active (may be NULL) receives the number of active elements. */
+int32_t hexa_prune_voids(hexa_handle* h, double min_stiffness_threshold, int64_t* n_active);
+
 /* Linear solve (K_ff(rho) + s I) u = f_free.  Replaces
  * GPUSolver.solve_system_gpu (GPUSolver.jl:467-477).  u_inout: nrhs*3*nN
  * host buffer, read as the warm start when prm->warm_start, written with

@@ -313,3 +313,45 @@ def test_reference_two_level_solve_matches_oracle_control_flow():
     assert r["iterations"] == info["iterations"]
     assert rel(u[0], uo) < 1e-6
     assert rel(u[0], ug) < 1e-6
+
+
+def test_device_void_pruning_matches_host_pruning():
+    """hexa_prune_voids (device) gives the same active set, node exclusion and
+    solution as the host-side MeshPruner path of the solve_system mirror
+    (Solver.jl:85-116), and leaves everything active when fewer than 1% of
+    the elements are voids."""
+    say("device pruning")
+    nx, ny, nz = 8, 4, 4
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    nodes = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.float32)
+    ek, ej, ei = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = (ei + (nx + 1) * (ej + (ny + 1) * ek)).ravel()
+    corners = [0, 1, 1 + (nx + 1), nx + 1]
+    off = corners + [c + (nx + 1) * (ny + 1) for c in corners]
+    elements = np.stack([base + o for o in off], 1) + 1
+    fixed, f, _ = cantilever_arrays(nx, ny, nz, 12)
+    rho = np.random.default_rng(2).uniform(0.2, 1.0, nx * ny * nz).astype(np.float32)
+    rho[[5, 6, 40, 41, 42, 77, 78, 100]] = 0.001
+    sp = {"tolerance": 1e-8, "max_iterations": 20000, "diagonal_shift_factor": 1e-10,
+          "preconditioner": "multigrid", "ke_mode": "fp64"}
+    cfg = {"geometry": {"nElem_x_computed": nx, "nElem_y_computed": ny, "nElem_z_computed": nz,
+                        "dx_computed": 1.0, "dy_computed": 1.0, "dz_computed": 1.0},
+           "solver_parameters": sp, "current_outer_iter": 5}
+    U, res, meth = hexa.solve_system(nodes, elements, 1.0, 0.3, fixed.reshape(-1, 3).astype(np.float32), f,
+                                     density=rho, config=cfg, min_stiffness_threshold=0.001)
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=0.0, penal=1.0, nu=0.3, ke_mode=hexa.KE_FP64,
+                     diag_mode=hexa.DIAG_X_ENTRY) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho.astype(np.float64))
+        n_active = s.prune_voids(0.001)
+        u, reps, st = s.solve(tol=float(np.float32(1e-8)), max_iter=20000,
+                              shift_factor=float(np.float32(1e-10)), precond=hexa.PRECOND_REF_MG)
+        assert n_active == int((rho > np.float32(0.001) * np.float32(1.01)).sum())
+        assert np.array_equal(u[0], U)
+        # under 1% voids: nothing is pruned
+        rho2 = rho.copy()
+        rho2[rho2 < 0.01] = 0.5
+        rho2[3] = 0.001
+        s.set_density(rho2.astype(np.float64))
+        assert s.prune_voids(0.001) == nx * ny * nz
+        assert s.prune_voids(-1.0) == nx * ny * nz

@@ -766,6 +766,8 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
     if (active_elem) {
         HX_TRY(dalloc(h, &h->d_active, (size_t)h->g.ne));
         HX_CHECK(hipMemcpyAsync(h->d_active, active_elem, (size_t)h->g.ne, hipMemcpyHostToDevice, st));
+    } else if (h->d_active) {
+        launch_fill<uint8_t>(st, h->g.ne, 1, h->d_active);   // a previous active set no longer applies
     }
     const uint8_t* act = active_elem ? h->d_active : nullptr;
     launch_node_mask(st, h->g, h->d_fixed, act, h->d_mask);
@@ -780,6 +782,37 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
     return HEXA_OK;
 }
 
+
+int32_t hexa_prune_voids(hexa_handle* h, double min_stiffness_threshold, int64_t* n_active) {
+    if (!h) { set_error("null handle"); return HEXA_E_INVALID; }
+    if (!h->have_problem || !h->d_rho) { set_error("hexa_set_problem and hexa_set_density come first"); return HEXA_E_INVALID; }
+    hipStream_t st = h->stream;
+    const int64_t ne = h->g.ne;
+    HX_TRY(dalloc(h, &h->d_active, (size_t)ne));
+    int64_t na = ne;
+    if (min_stiffness_threshold >= 0.0) {
+        // Solver.jl:86-89: Float32 threshold * 1.01f, strict comparison
+        const float thr = (float)min_stiffness_threshold * 1.01f;
+        HX_TRY(ensure_solver_vectors(h));
+        launch_active_from_rho(st, ne, h->d_rho, thr, h->d_active, h->d_partials);
+        launch_final_sum(st, 1, h->d_partials, h->d_scal + 5);
+        HX_TRY(check_launch());
+        double cnt = 0.0;
+        HX_CHECK(hipMemcpyAsync(&cnt, h->d_scal + 5, sizeof(double), hipMemcpyDeviceToHost, st));
+        HX_CHECK(hipStreamSynchronize(st));
+        if (cnt == 0.0) { set_error("MeshPruner: no active elements"); return HEXA_E_INVALID; }
+        na = (int64_t)cnt;
+        // prune only when fewer than 99% of the elements are active (:90)
+        if ((double)na >= (double)ne * 0.99) na = ne;
+    }
+    if (na == ne) launch_fill<uint8_t>(st, ne, 1, h->d_active);
+    launch_node_mask(st, h->g, h->d_fixed, h->d_active, h->d_mask);
+    HX_TRY(check_launch());
+    h->precond_ready = -1;
+    h->refmg_ready = false;
+    if (n_active) *n_active = na;
+    return update_factor(h);
+}
 
 int32_t hexa_set_density(hexa_handle* h, const double* rho) {
     if (!h || !rho) { set_error("null argument"); return HEXA_E_INVALID; }

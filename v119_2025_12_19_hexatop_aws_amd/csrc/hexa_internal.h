@@ -132,6 +132,9 @@ void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, cons
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf);
+// active[e] = (float)rho[e] > thr; RED_BLOCKS per-block counts in partials
+void launch_active_from_rho(hipStream_t s, int64_t ne, const double* rho, float thr, uint8_t* active,
+                            double* partials);
 void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
                         const double* kd8, double* dcinv);
 void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,

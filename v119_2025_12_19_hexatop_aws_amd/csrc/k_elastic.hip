@@ -214,6 +214,24 @@ __global__ void k_node_mask(Grid3 g, const uint8_t* __restrict__ fixed_dof,
     }
 }
 
+// Void pruning on the device (R/src/Solvers/Solver.jl:85-91): an element
+// stays active when its Float32 density exceeds the Float32 threshold; the
+// per-block counts of active elements go to partials[blockIdx.x].
+__global__ __launch_bounds__(256) void k_active_from_rho(int64_t ne, const double* __restrict__ rho, float thr,
+                                                         uint8_t* __restrict__ active, double* __restrict__ partials) {
+    __shared__ double lds[4];
+    double cnt = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint8_t a = (float)rho[e] > thr ? 1 : 0;
+        active[e] = a;
+        cnt += a;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
 // Coarse stiffness multiplier (rediscretisation): sum of the 8 children / 8.
 // A coarse grid has ceil(n/2) elements per axis; on an odd fine axis the last
 // coarse element overhangs the box by one fine layer, whose missing children
@@ -292,6 +310,11 @@ void launch_diag(hipStream_t s, const Grid3& g, const double* Ke, const T* fac, 
 void launch_node_mask(hipStream_t s, const Grid3& g, const uint8_t* fixed_dof, const uint8_t* active,
                       uint8_t* mask) {
     k_node_mask<<<grid_for(g.nn), 256, 0, s>>>(g, fixed_dof, active, mask);
+}
+
+void launch_active_from_rho(hipStream_t s, int64_t ne, const double* rho, float thr, uint8_t* active,
+                            double* partials) {
+    k_active_from_rho<<<RED_BLOCKS, 256, 0, s>>>(ne, rho, thr, active, partials);
 }
 
 template <typename T>

@@ -144,6 +144,7 @@ SIGNATURES = {
     "hexa_set_problem": (C.c_int32, [_H, _P, _P, _P]),
     "hexa_set_density": (C.c_int32, [_H, _P]),
     "hexa_set_density_dev": (C.c_int32, [_H, _P]),
+    "hexa_prune_voids": (C.c_int32, [_H, C.c_double, C.POINTER(C.c_int64)]),
     "hexa_solve": (C.c_int32, [_H, _P, C.POINTER(SolveParams), _P]),
     "hexa_get_displacement": (C.c_int32, [_H, _P]),
     "hexa_displacement_dev": (C.c_int32, [_H, C.POINTER(C.c_void_p)]),
@@ -287,6 +288,14 @@ class Solver:
         st = lib().hexa_solve(self._h, _ptr(u), C.byref(prm), C.cast(reps, C.c_void_p))
         _check(st, allow=(OK, E_NOT_CONVERGED, E_NAN_INPUT))
         return u, [r.as_dict() for r in reps], st
+
+    def prune_voids(self, min_stiffness_threshold):
+        """Device-side void pruning (Solver.jl:85-116); returns the number of
+        active elements (nE when nothing was pruned); a negative threshold
+        makes every element active again."""
+        n = C.c_int64()
+        _check(lib().hexa_prune_voids(self._h, float(min_stiffness_threshold), C.byref(n)))
+        return n.value
 
     def matvec(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
