@@ -271,6 +271,49 @@ int32_t hexa_update_density(hexa_handle* h, float* density, const float* l1, con
                             const float* original_density, const hexa_update_params* p,
                             hexa_update_report* rep);
 
+/* The reference's outer iteration kept on the device (Main.jl:439-493):
+ * solve (K(rho) + s I) u = f with the handle's density (after void pruning
+ * when prune_threshold >= 0, as Solver.jl:85-116), the stress l1 of that u
+ * (Stress.jl:159-285, material E), and update_density! (TopOpt.jl:211-482)
+ * of the handle's density, which then becomes the handle's density.  No
+ * field crosses PCIe: the protected flags and original densities are set
+ * once with hexa_set_update_fields; density_out (nE Float32, may be NULL)
+ * receives a copy of the new density.  Returns the solve's status. */
+int32_t hexa_set_update_fields(hexa_handle* h, const uint8_t* protected_mask, const float* original_density);
+int32_t hexa_ref_step(hexa_handle* h, const hexa_solve_params* solve, double E, double prune_threshold,
+                      const hexa_update_params* update, hexa_solve_report* solve_report,
+                      hexa_update_report* update_report, float* density_out);
+
+/* ---- files and regridding from device fields ----------------------------
+ * hexa_write_vti: VTK XML image data (.vti appended if missing) with the
+ * layout of ExportVTK.export_solution_vti (ExportVTK.jl:16-111): Density,
+ * L1_Stress, VonMises and, when write_vectors, PrincipalValues and the
+ * max/min principal directions, computed from the handle's device solution
+ * (stress with material E).  density: nE Float32 host array, or NULL for
+ * the handle's density.  Origin and the handle's spacing go to the header.
+ * hexa_write_bin: the .bin checkpoint of Postprocessing.export_binary_for_web
+ * (Postprocessing.jl:370-471): elements with density >= threshold, strided
+ * down to max_export_cells when > 0; full_field = 1 writes every element
+ * instead, which is what Configuration.load_checkpoint (Configuration.jl:
+ * 32-78) restarts from.  json: the config JSON (NULL = "{}"). */
+int32_t hexa_write_vti(hexa_handle* h, const char* path, const float* density, double E, double origin_x,
+                       double origin_y, double origin_z, int32_t write_vectors);
+int32_t hexa_write_bin(hexa_handle* h, const char* path, const float* density, double E, int32_t iter,
+                       float radius, float threshold, int64_t max_export_cells, int32_t full_field,
+                       const char* json);
+/* Density (and optional alpha) remap onto a refined grid, device arrays
+ * (MeshRefiner.jl:185-237): new element centroids mapped into the old cell
+ * index space of a domain len_x * len_y * len_z, trilinear interpolation of
+ * the clamped neighbours, hard != 0 projects (> 0.4 -> 1, < 0.2 -> 1e-4);
+ * alpha takes the nearest-lower neighbour.  stream: a hipStream_t or NULL. */
+int32_t hexa_regrid_density(const hexa_grid* old_grid, const float* d_old_density, const float* d_old_alpha,
+                            const hexa_grid* new_grid, double len_x, double len_y, double len_z, int32_t hard,
+                            float* d_new_density, float* d_new_alpha, void* stream);
+/* The same with host arrays (device copies made and freed inside). */
+int32_t hexa_regrid_density_host(const hexa_grid* old_grid, const float* old_density, const float* old_alpha,
+                                 const hexa_grid* new_grid, double len_x, double len_y, double len_z, int32_t hard,
+                                 float* new_density, float* new_alpha);
+
 /* ---- job server (no reference counterpart) ------------------------------
  * Topology-optimisation jobs of mixed mesh sizes served on one GPU.  Each
  * job: a box of nx*ny*nz unit bricks, all DOFs of one face held

@@ -624,6 +624,45 @@ void or_refmg_apply(int nx, int ny, int nz, const double* kd8, const uint8_t* pr
     free(dc);
 }
 
+/* ------------------------------------------------------------ regridding */
+/* Density remap onto a refined grid, R/src/Mesh/MeshRefiner.jl:185-237 with
+ * interpolate_trilinear_projected (:44-75), in the reference's Float32
+ * operation order (this file is built without FMA contraction).  Grids are
+ * element counts; lengths are the domain extents; alpha may be NULL. */
+void or_regrid(int onx, int ony, int onz, int nnx, int nny, int nnz, float ndx, float ndy, float ndz,
+               float lx, float ly, float lz, int hard, const float* rho, const float* alpha, float* out,
+               float* aout) {
+    const int64_t n_new = (int64_t)nnx * nny * nnz, sl = (int64_t)onx * ony;
+    for (int64_t e = 0; e < n_new; ++e) {
+        const int64_t ix = e % nnx + 1, iy = (e / nnx) % nny + 1, iz = e / ((int64_t)nnx * nny) + 1;
+        const float cx = ((float)ix - 0.5f) * ndx, cy = ((float)iy - 0.5f) * ndy, cz = ((float)iz - 0.5f) * ndz;
+        float u = (cx / lx) * (float)onx, v = (cy / ly) * (float)ony, w = (cz / lz) * (float)onz;
+        u = u < 0.5f ? 0.5f : (u > (float)onx - 0.5f ? (float)onx - 0.5f : u);
+        v = v < 0.5f ? 0.5f : (v > (float)ony - 0.5f ? (float)ony - 0.5f : v);
+        w = w < 0.5f ? 0.5f : (w > (float)onz - 0.5f ? (float)onz - 0.5f : w);
+        const int64_t u0 = (int64_t)floorf(u - 0.5f), v0 = (int64_t)floorf(v - 0.5f), w0 = (int64_t)floorf(w - 0.5f);
+        const float tx = (u - 0.5f) - (float)u0, ty = (v - 0.5f) - (float)v0, tz = (w - 0.5f) - (float)w0;
+#define CL(a, n) ((a) < 1 ? (int64_t)1 : ((a) > (n) ? (int64_t)(n) : (a)))
+        const int64_t i0 = CL(u0 + 1, onx), i1 = CL(u0 + 2, onx), j0 = CL(v0 + 1, ony), j1 = CL(v0 + 2, ony),
+                      k0 = CL(w0 + 1, onz), k1 = CL(w0 + 2, onz);
+#undef CL
+#define AT(i, j, k) rho[((i) - 1) + ((j) - 1) * onx + ((k) - 1) * sl]
+        const float ctx = tx < 0.f ? 0.f : (tx > 1.f ? 1.f : tx), cty = ty < 0.f ? 0.f : (ty > 1.f ? 1.f : ty),
+                    ctz = tz < 0.f ? 0.f : (tz > 1.f ? 1.f : tz);
+        const float c00 = AT(i0, j0, k0) * (1 - ctx) + AT(i1, j0, k0) * ctx;
+        const float c10 = AT(i0, j1, k0) * (1 - ctx) + AT(i1, j1, k0) * ctx;
+        const float c01 = AT(i0, j0, k1) * (1 - ctx) + AT(i1, j0, k1) * ctx;
+        const float c11 = AT(i0, j1, k1) * (1 - ctx) + AT(i1, j1, k1) * ctx;
+#undef AT
+        const float c0 = c00 * (1 - cty) + c10 * cty;
+        const float c1 = c01 * (1 - cty) + c11 * cty;
+        float val = c0 * (1 - ctz) + c1 * ctz;
+        if (hard) val = val > 0.4f ? 1.0f : (val < 0.2f ? 0.0001f : val);
+        out[e] = val;
+        if (aout) aout[e] = alpha ? alpha[(i0 - 1) + (j0 - 1) * onx + (k0 - 1) * sl] : 0.0f;
+    }
+}
+
 /* ------------------------------------------------------- Helmholtz filter */
 static void helm_apply(int nx, int ny, int nz, const double* Ae, const double* x, double* y) {
     int64_t nN = (int64_t)(nx + 1) * (ny + 1) * (nz + 1), nE = (int64_t)nx * ny * nz;

@@ -66,6 +66,8 @@ def lib():
         L.or_pcg_gpu_semantics.argtypes = [C.c_int] * 3 + [
             _dp, _dp, _u8, _dp, C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
             C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int]
+        L.or_regrid.restype = None
+        L.or_regrid.argtypes = [C.c_int] * 6 + [C.c_float] * 6 + [C.c_int, _fp, C.c_void_p, _fp, C.c_void_p]
         L.or_refmg_apply.restype = None
         L.or_refmg_apply.argtypes = [C.c_int] * 3 + [_dp, C.c_void_p, _u8, _dp, _dp, _dp]
         L.or_filter.restype = C.c_int
@@ -144,6 +146,17 @@ def pcg_cpu(grid, Ke, factor, fixed, f, tol=1e-8, max_iter=10000, shift_factor=0
                           np.ascontiguousarray(fixed, dtype=np.uint8), _f64(f), tol, max_iter,
                           shift_factor, x, C.byref(rel), C.byref(sh), nchunks)
     return x, rel.value, it, sh.value
+
+
+def regrid(old_dims, new_dims, new_h, lengths, rho, hard=False, alpha=None):
+    """MeshRefiner density remap (MeshRefiner.jl:185-237), Float32."""
+    out = np.zeros(int(np.prod(new_dims)), dtype=np.float32)
+    aout = np.zeros_like(out) if alpha is not None else None
+    a = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float32)
+    lib().or_regrid(*[int(v) for v in old_dims], *[int(v) for v in new_dims], *[float(v) for v in new_h],
+                    *[float(v) for v in lengths], int(bool(hard)), np.ascontiguousarray(rho, dtype=np.float32),
+                    _ptr(a), out, _ptr(aout))
+    return out if alpha is None else (out, aout)
 
 
 def refmg_kd8(dx=1.0, dy=1.0, dz=1.0):

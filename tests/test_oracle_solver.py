@@ -318,3 +318,32 @@ def test_oracle_reference_two_level_preconditioner(grid, prune):
         i = n % (nx + 1)
         if i <= 2 * cx:
             assert abs(lin_f[n] - 0.5 * i) < 1e-14 or (n // (nx + 1)) % (ny + 1) > 2 * cy or n // ((nx + 1) * (ny + 1)) > 2 * cz
+
+
+def test_oracle_regrid_closed_forms():
+    """MeshRefiner density remap (oracle): the same grid maps every element to
+    itself bit for bit; a field linear in x is reproduced exactly at new
+    centroids whose old-grid neighbours are unclamped; constants stay
+    constant; the hard projection snaps > 0.4 to 1 and < 0.2 to 1e-4."""
+    rng = np.random.default_rng(3)
+    od = (6, 4, 3)
+    L = (6.0, 4.0, 3.0)
+    rho = rng.uniform(0.0, 1.0, int(np.prod(od))).astype(np.float32)
+    same = oracle.regrid(od, od, (1.0, 1.0, 1.0), L, rho)
+    assert np.array_equal(same, rho)
+    nd, nh = (12, 8, 6), (0.5, 0.5, 0.5)
+    ones = oracle.regrid(od, nd, nh, L, np.full(rho.size, 0.7, np.float32))
+    assert np.all(ones == np.float32(0.7))
+    ox = np.array([i for k in range(od[2]) for j in range(od[1]) for i in range(od[0])], np.float32)
+    lin = 0.1 + 0.1 * ox                       # value at old cell centre x = i + 0.5
+    out = oracle.regrid(od, nd, nh, L, lin.astype(np.float32))
+    for e in range(out.size):
+        ix = e % nd[0]
+        x = (ix + 0.5) * nh[0]                 # new centroid
+        if 0.5 <= x <= od[0] - 0.5:
+            assert abs(out[e] - (0.1 + 0.1 * (x - 0.5))) < 1e-6
+    hard = oracle.regrid(od, nd, nh, L, rho, hard=True)
+    soft = oracle.regrid(od, nd, nh, L, rho)
+    assert np.all(hard[soft > 0.4] == 1.0) and np.all(hard[soft < 0.2] == np.float32(1e-4))
+    mid = (soft >= 0.2) & (soft <= 0.4)
+    assert np.array_equal(hard[mid], soft[mid])

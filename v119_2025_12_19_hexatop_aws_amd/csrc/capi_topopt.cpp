@@ -294,31 +294,32 @@ int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc
     return ss;
 }
 
-int32_t hexa_update_density(hexa_handle* h, float* density, const float* l1, const uint8_t* prot,
-                            const float* orig, const hexa_update_params* P, hexa_update_report* rep) {
-    if (!h || !density || !l1 || !prot || !orig || !P) { set_error("null argument"); return HEXA_E_INVALID; }
-    if (P->n_iters < 1 || !(P->l1_allow != 0) || !(P->E != 0)) { set_error("invalid update parameters"); return HEXA_E_INVALID; }
+}  // extern "C"
+
+namespace {
+
+// update_density! (TopOpt.jl:211-482) on device fields: density d_f0 (in/out),
+// l1 d_f1, original density d_f2, protected flags d_u0; the element work
+// arrays must exist (ensure_elem).
+int32_t update_core(hexa_handle* h, const hexa_update_params* P, hexa_update_report* rep) {
     const int64_t ne = h->g.ne;
     hexa_update_report R{};
-    for (int64_t e = 0; e < ne; ++e)
-        if (std::isnan(l1[e])) {  // TopOpt.jl:230-236
-            R.skipped = 1;
-            if (rep) *rep = R;
-            return HEXA_OK;
-        }
-    HX_TRY(ensure_elem(h));
-    HX_TRY(dalloc(h, &h->d_f0, (size_t)ne));
-    HX_TRY(dalloc(h, &h->d_f1, (size_t)ne));
-    HX_TRY(dalloc(h, &h->d_f2, (size_t)ne));
-    HX_TRY(dalloc(h, &h->d_u0, (size_t)ne));
+    hipStream_t st = h->stream;
+    // TopOpt.jl:230-236: NaN in l1 skips the update
+    launch_nan_count(st, ne, (const float*)h->d_f1, h->d_partials);
+    launch_final_sum(st, 1, h->d_partials, h->d_scal);
+    HX_TRY(check_launch());
+    double nanc = 0.0;
+    HX_CHECK(hipMemcpyAsync(&nanc, h->d_scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    if (nanc > 0.0) {
+        R.skipped = 1;
+        if (rep) *rep = R;
+        return HEXA_OK;
+    }
     HX_TRY(dalloc(h, &h->d_u1, (size_t)ne));
     HX_TRY(dalloc(h, &h->d_hist, 256));
     HX_TRY(dalloc(h, &h->d_ties, (size_t)(2 * RED_BLOCKS)));
-    hipStream_t st = h->stream;
-    HX_CHECK(hipMemcpyAsync(h->d_f0, density, sizeof(float) * ne, hipMemcpyHostToDevice, st));
-    HX_CHECK(hipMemcpyAsync(h->d_f1, l1, sizeof(float) * ne, hipMemcpyHostToDevice, st));
-    HX_CHECK(hipMemcpyAsync(h->d_f2, orig, sizeof(float) * ne, hipMemcpyHostToDevice, st));
-    HX_CHECK(hipMemcpyAsync(h->d_u0, prot, (size_t)ne, hipMemcpyHostToDevice, st));
     const float mind = (float)P->min_density, maxd = (float)P->max_density;
     launch_upd_proposed(st, ne, h->d_f1, h->d_u0, h->d_f2, (float)P->l1_allow, (float)P->E, mind, h->d_e0);
     const float R_len = radius_schedule(P, h->grid.dx, h->grid.dy, h->grid.dz);
@@ -400,12 +401,98 @@ int32_t hexa_update_density(hexa_handle* h, float* density, const float* l1, con
     launch_cull(st, ne, h->d_u1, key_t, ties, h->d_ties + RED_BLOCKS, mind, h->d_f0);
     launch_upd_finish(st, ne, h->d_u0, h->d_f2, mind, P->hard, h->d_f0);
     HX_TRY(check_launch());
-    HX_CHECK(hipMemcpyAsync(density, h->d_f0, sizeof(float) * ne, hipMemcpyDeviceToHost, st));
     HX_CHECK(hipStreamSynchronize(st));
     R.R_length = R_len;
     R.threshold = thr;
     if (rep) *rep = R;
     return HEXA_OK;
+}
+
+int32_t upload_update_inputs(hexa_handle* h, const float* l1, const uint8_t* prot, const float* orig) {
+    const int64_t ne = h->g.ne;
+    hipStream_t st = h->stream;
+    HX_TRY(dalloc(h, &h->d_f1, (size_t)ne));
+    HX_TRY(dalloc(h, &h->d_f2, (size_t)ne));
+    HX_TRY(dalloc(h, &h->d_u0, (size_t)ne));
+    if (l1) HX_CHECK(hipMemcpyAsync(h->d_f1, l1, sizeof(float) * ne, hipMemcpyHostToDevice, st));
+    HX_CHECK(hipMemcpyAsync(h->d_f2, orig, sizeof(float) * ne, hipMemcpyHostToDevice, st));
+    HX_CHECK(hipMemcpyAsync(h->d_u0, prot, (size_t)ne, hipMemcpyHostToDevice, st));
+    return HEXA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hexa_update_density(hexa_handle* h, float* density, const float* l1, const uint8_t* prot,
+                            const float* orig, const hexa_update_params* P, hexa_update_report* rep) {
+    if (!h || !density || !l1 || !prot || !orig || !P) { set_error("null argument"); return HEXA_E_INVALID; }
+    if (P->n_iters < 1 || !(P->l1_allow != 0) || !(P->E != 0)) { set_error("invalid update parameters"); return HEXA_E_INVALID; }
+    const int64_t ne = h->g.ne;
+    HX_TRY(ensure_elem(h));
+    HX_TRY(ensure_solver_vectors(h));
+    HX_TRY(dalloc(h, &h->d_f0, (size_t)ne));
+    HX_TRY(upload_update_inputs(h, l1, prot, orig));
+    hipStream_t st = h->stream;
+    HX_CHECK(hipMemcpyAsync(h->d_f0, density, sizeof(float) * ne, hipMemcpyHostToDevice, st));
+    hexa_update_report R{};
+    HX_TRY(update_core(h, P, &R));
+    if (!R.skipped) {
+        HX_CHECK(hipMemcpyAsync(density, h->d_f0, sizeof(float) * ne, hipMemcpyDeviceToHost, st));
+        HX_CHECK(hipStreamSynchronize(st));
+    }
+    if (rep) *rep = R;
+    return HEXA_OK;
+}
+
+int32_t hexa_set_update_fields(hexa_handle* h, const uint8_t* prot, const float* orig) {
+    if (!h || !prot || !orig) { set_error("null argument"); return HEXA_E_INVALID; }
+    const int64_t ne = h->g.ne;
+    HX_TRY(dalloc(h, &h->d_prot, (size_t)ne));
+    HX_TRY(dalloc(h, &h->d_orig, (size_t)ne));
+    HX_CHECK(hipMemcpyAsync(h->d_prot, prot, (size_t)ne, hipMemcpyHostToDevice, h->stream));
+    HX_CHECK(hipMemcpyAsync(h->d_orig, orig, sizeof(float) * ne, hipMemcpyHostToDevice, h->stream));
+    HX_CHECK(hipStreamSynchronize(h->stream));
+    h->have_update_fields = true;
+    return HEXA_OK;
+}
+
+int32_t hexa_ref_step(hexa_handle* h, const hexa_solve_params* sp, double E, double prune_threshold,
+                      const hexa_update_params* P, hexa_solve_report* srep, hexa_update_report* urep,
+                      float* density_out) {
+    if (!h || !P) { set_error("null argument"); return HEXA_E_INVALID; }
+    if (!h->have_update_fields) { set_error("hexa_set_update_fields comes first"); return HEXA_E_INVALID; }
+    if (P->n_iters < 1 || !(P->l1_allow != 0) || !(P->E != 0)) { set_error("invalid update parameters"); return HEXA_E_INVALID; }
+    const int64_t ne = h->g.ne;
+    hipStream_t st = h->stream;
+    // 1. solve with the handle's density (pruned first when asked)
+    if (prune_threshold >= 0.0) HX_TRY(hexa_prune_voids(h, prune_threshold, nullptr));
+    int32_t ss = hexa_solve(h, nullptr, sp, srep);
+    if (ss != HEXA_OK && ss != HEXA_E_NOT_CONVERGED) return ss;
+    // 2. stress l1 of the device solution (Stress.jl:159-285)
+    HX_TRY(ensure_elem(h));
+    HX_TRY(dalloc(h, &h->d_stress, (size_t)(17 * ne)));
+    HX_TRY(dalloc(h, &h->d_f0, (size_t)ne));
+    HX_TRY(dalloc(h, &h->d_f1, (size_t)ne));
+    HX_TRY(dalloc(h, &h->d_f2, (size_t)ne));
+    HX_TRY(dalloc(h, &h->d_u0, (size_t)ne));
+    launch_stress(st, h->g, h->grid.dx, h->grid.dy, h->grid.dz, E, h->opt.nu, h->d_x, h->d_rho, nullptr, nullptr,
+                  h->d_f1, nullptr, nullptr, nullptr);
+    // 3. update_density! on device copies of the density, flags and originals
+    launch_convert<double, float>(st, ne, h->d_rho, h->d_f0);
+    HX_CHECK(hipMemcpyAsync(h->d_f2, h->d_orig, sizeof(float) * ne, hipMemcpyDeviceToDevice, st));
+    HX_CHECK(hipMemcpyAsync(h->d_u0, h->d_prot, (size_t)ne, hipMemcpyDeviceToDevice, st));
+    HX_TRY(check_launch());
+    hexa_update_report R{};
+    HX_TRY(update_core(h, P, &R));
+    if (urep) *urep = R;
+    if (R.skipped) return ss;
+    // 4. the new density becomes the handle's density
+    launch_convert<float, double>(st, ne, h->d_f0, h->d_rho);
+    HX_TRY(check_launch());
+    if (density_out) HX_CHECK(hipMemcpyAsync(density_out, h->d_f0, sizeof(float) * ne, hipMemcpyDeviceToHost, st));
+    HX_TRY(update_factor(h));
+    return ss;
 }
 
 }  // extern "C"

@@ -96,6 +96,11 @@ struct hexa_handle {
     double w_radius = -1.0;         // radius the filter weights d_w were computed for
     double* d_w = nullptr;
     float *d_f0 = nullptr, *d_f1 = nullptr, *d_f2 = nullptr;
+    // protected mask and original density of the reference update, resident
+    // for the device loop (hexa_set_update_fields)
+    uint8_t* d_prot = nullptr;
+    float* d_orig = nullptr;
+    bool have_update_fields = false;
     uint8_t *d_u0 = nullptr, *d_u1 = nullptr;
     void* d_bracket = nullptr;
     unsigned long long* d_hist = nullptr;

@@ -178,7 +178,11 @@ void launch_oc(hipStream_t s, int64_t n, const double* dc, const double* dv, con
                double* partials, double* out4);
 void launch_upd_proposed(hipStream_t s, int64_t n, const float* l1, const uint8_t* prot, const float* orig, float allow,
                          float E, float mind, double* prop);
+void launch_regrid(hipStream_t s, int onx, int ony, int onz, int nnx, int nny, int nnz, float ndx, float ndy,
+                   float ndz, float lx, float ly, float lz, int hard, const float* rho, const float* alpha, float* out,
+                   float* aout);
 void launch_nan_count(hipStream_t s, int64_t n, const double* x, double* partials);
+void launch_nan_count(hipStream_t s, int64_t n, const float* x, double* partials);
 void launch_upd_apply(hipStream_t s, int64_t n, const double* filt, const double* prop, int use_prop,
                       const uint8_t* prot, float mind, float maxd, float* dens, double* partials);
 void launch_upd_candidates(hipStream_t s, int64_t n, const uint8_t* prot, const float* orig, float mind, float maxd,

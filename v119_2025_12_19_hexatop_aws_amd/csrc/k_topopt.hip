@@ -328,7 +328,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_upd_apply(int64_t n, const doub
     if (threadIdx.x == 0) { partials[blockIdx.x] = ch; partials[RED_BLOCKS + blockIdx.x] = act; }
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_nan_count(int64_t n, const double* __restrict__ x,
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_nan_count(int64_t n, const T* __restrict__ x,
                                                            double* __restrict__ partials) {
     __shared__ double lds[RED_THREADS / 64];
     double c = 0.0;
@@ -440,6 +441,47 @@ __global__ void k_upd_finish(int64_t n, const uint8_t* __restrict__ prot, const 
     }
 }
 
+// Density remap onto a refined grid (R/src/Mesh/MeshRefiner.jl:185-237):
+// the new element's centroid in the old grid's continuous cell-index space,
+// clamped to [0.5, n - 0.5]; trilinear interpolation of the 8 surrounding
+// old cells (clamped indices) with the clamped weights of
+// interpolate_trilinear_projected (:44-75), optionally projected (> 0.4 -> 1,
+// < 0.2 -> 1e-4); the alpha field takes the (0,0,0) neighbour.  Float32
+// arithmetic in the reference's order, with contraction into FMAs off, so the
+// result is the reference's bit for bit.
+__global__ void k_regrid(int onx, int ony, int onz, int nnx, int nny, int nnz, float ndx, float ndy, float ndz,
+                         float lx, float ly, float lz, int hard, const float* __restrict__ rho,
+                         const float* __restrict__ alpha, float* __restrict__ out, float* __restrict__ aout) {
+#pragma clang fp contract(off)
+    const int64_t n_new = (int64_t)nnx * nny * nnz;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n_new; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ix = e % nnx + 1, iy = (e / nnx) % nny + 1, iz = e / ((int64_t)nnx * nny) + 1;
+        const float cx = ((float)ix - 0.5f) * ndx, cy = ((float)iy - 0.5f) * ndy, cz = ((float)iz - 0.5f) * ndz;
+        const float u = fminf(fmaxf((cx / lx) * (float)onx, 0.5f), (float)onx - 0.5f);
+        const float v = fminf(fmaxf((cy / ly) * (float)ony, 0.5f), (float)ony - 0.5f);
+        const float w = fminf(fmaxf((cz / lz) * (float)onz, 0.5f), (float)onz - 0.5f);
+        const int64_t u0 = (int64_t)floorf(u - 0.5f), v0 = (int64_t)floorf(v - 0.5f), w0 = (int64_t)floorf(w - 0.5f);
+        const float tx = (u - 0.5f) - (float)u0, ty = (v - 0.5f) - (float)v0, tz = (w - 0.5f) - (float)w0;
+        auto cl = [](int64_t a, int64_t n) { return a < 1 ? (int64_t)1 : (a > n ? n : a); };
+        const int64_t i0 = cl(u0 + 1, onx), i1 = cl(u0 + 2, onx), j0 = cl(v0 + 1, ony), j1 = cl(v0 + 2, ony),
+                      k0 = cl(w0 + 1, onz), k1 = cl(w0 + 2, onz);
+        const int64_t sl = (int64_t)onx * ony;
+        auto at = [&](int64_t i, int64_t j, int64_t k) { return rho[(i - 1) + (j - 1) * onx + (k - 1) * sl]; };
+        const float ctx = fminf(fmaxf(tx, 0.0f), 1.0f), cty = fminf(fmaxf(ty, 0.0f), 1.0f),
+                    ctz = fminf(fmaxf(tz, 0.0f), 1.0f);
+        const float c00 = at(i0, j0, k0) * (1 - ctx) + at(i1, j0, k0) * ctx;
+        const float c10 = at(i0, j1, k0) * (1 - ctx) + at(i1, j1, k0) * ctx;
+        const float c01 = at(i0, j0, k1) * (1 - ctx) + at(i1, j0, k1) * ctx;
+        const float c11 = at(i0, j1, k1) * (1 - ctx) + at(i1, j1, k1) * ctx;
+        const float c0 = c00 * (1 - cty) + c10 * cty;
+        const float c1 = c01 * (1 - cty) + c11 * cty;
+        float val = c0 * (1 - ctz) + c1 * ctz;
+        if (hard) val = val > 0.4f ? 1.0f : (val < 0.2f ? 0.0001f : val);
+        out[e] = val;
+        if (aout) aout[e] = alpha ? alpha[(i0 - 1) + (j0 - 1) * onx + (k0 - 1) * sl] : 0.0f;
+    }
+}
+
 }  // namespace
 
 void launch_stress(hipStream_t s, const Grid3& g, double dx, double dy, double dz, double E, double nu,
@@ -470,7 +512,10 @@ void launch_upd_proposed(hipStream_t s, int64_t n, const float* l1, const uint8_
     k_upd_proposed<<<grid_for(n), 256, 0, s>>>(n, l1, prot, orig, allow, E, mind, prop);
 }
 void launch_nan_count(hipStream_t s, int64_t n, const double* x, double* partials) {
-    k_nan_count<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, x, partials);
+    k_nan_count<double><<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, x, partials);
+}
+void launch_nan_count(hipStream_t s, int64_t n, const float* x, double* partials) {
+    k_nan_count<float><<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, x, partials);
 }
 void launch_upd_apply(hipStream_t s, int64_t n, const double* filt, const double* prop, int use_prop,
                       const uint8_t* prot, float mind, float maxd, float* dens, double* partials) {
@@ -497,6 +542,16 @@ void launch_cull(hipStream_t s, int64_t n, const uint8_t* cand, uint32_t key_t, 
 void launch_upd_finish(hipStream_t s, int64_t n, const uint8_t* prot, const float* orig, float mind, int hard,
                        float* dens) {
     k_upd_finish<<<grid_for(n), 256, 0, s>>>(n, prot, orig, mind, hard, dens);
+}
+
+void launch_regrid(hipStream_t s, int onx, int ony, int onz, int nnx, int nny, int nnz, float ndx, float ndy,
+                   float ndz, float lx, float ly, float lz, int hard, const float* rho, const float* alpha, float* out,
+                   float* aout) {
+    const int64_t n = (int64_t)nnx * nny * nnz;
+    int64_t b = (n + 255) / 256;
+    if (b > 65535 * 8) b = 65535 * 8;
+    k_regrid<<<(unsigned)(b < 1 ? 1 : b), 256, 0, s>>>(onx, ony, onz, nnx, nny, nnz, ndx, ndy, ndz, lx, ly, lz, hard,
+                                                       rho, alpha, out, aout);
 }
 
 }  // namespace hexa

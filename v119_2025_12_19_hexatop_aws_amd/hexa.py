@@ -161,6 +161,16 @@ SIGNATURES = {
                                    C.POINTER(C.c_double)]),
     "hexa_oc_step": (C.c_int32, [_H, _P, C.POINTER(OcParams), C.POINTER(OcReport)]),
     "hexa_update_density": (C.c_int32, [_H, _P, _P, _P, _P, C.POINTER(UpdateParams), C.POINTER(UpdateReport)]),
+    "hexa_set_update_fields": (C.c_int32, [_H, _P, _P]),
+    "hexa_write_vti": (C.c_int32, [_H, C.c_char_p, _P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "hexa_write_bin": (C.c_int32, [_H, C.c_char_p, _P, C.c_double, C.c_int32, C.c_float, C.c_float, C.c_int64,
+                                   C.c_int32, C.c_char_p]),
+    "hexa_regrid_density": (C.c_int32, [C.POINTER(Grid), _P, _P, C.POINTER(Grid), C.c_double, C.c_double,
+                                        C.c_double, C.c_int32, _P, _P, _P]),
+    "hexa_regrid_density_host": (C.c_int32, [C.POINTER(Grid), _P, _P, C.POINTER(Grid), C.c_double, C.c_double,
+                                             C.c_double, C.c_int32, _P, _P]),
+    "hexa_ref_step": (C.c_int32, [_H, C.POINTER(SolveParams), C.c_double, C.c_double, C.POINTER(UpdateParams),
+                                  C.POINTER(SolveReport), C.POINTER(UpdateReport), _P]),
     "hexa_estimate_bytes": (C.c_int64, [C.POINTER(Grid), C.POINTER(Options)]),
     "hexa_server_create": (C.c_int32, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
     "hexa_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_int64)]),
@@ -373,6 +383,38 @@ class Solver:
                                          _ptr(f32(original)), C.byref(p), C.byref(rep)))
         return d, rep.as_dict()
 
+    def set_update_fields(self, protected, original):
+        """Protected flags and original densities of the reference update,
+        resident on the device for ref_step."""
+        self._upd = (np.ascontiguousarray(protected, dtype=np.uint8).reshape(-1),
+                     np.ascontiguousarray(original, dtype=np.float32).reshape(-1))
+        _check(lib().hexa_set_update_fields(self._h, _ptr(self._upd[0]), _ptr(self._upd[1])))
+
+    def ref_step(self, E, update, prune_threshold=-1.0, want_density=True, **solve_params):
+        """One outer iteration of the reference loop on the device: solve,
+        stress l1, update_density!.  Returns (density or None, solve report,
+        update report, status)."""
+        sp = default_params(**solve_params)
+        up = UpdateParams(**update)
+        sr, ur = SolveReport(), UpdateReport()
+        d = np.zeros(self.ne, dtype=np.float32) if want_density else None
+        st = lib().hexa_ref_step(self._h, C.byref(sp), float(E), float(prune_threshold), C.byref(up),
+                                 C.byref(sr), C.byref(ur), _ptr(d))
+        _check(st, allow=(OK, E_NOT_CONVERGED))
+        return d, sr.as_dict(), ur.as_dict(), st
+
+    def write_vti(self, path, E=1.0, density=None, origin=(0.0, 0.0, 0.0), write_vectors=False):
+        d = None if density is None else np.ascontiguousarray(density, dtype=np.float32).reshape(-1)
+        _check(lib().hexa_write_vti(self._h, str(path).encode(), _ptr(d), float(E), *[float(v) for v in origin],
+                                    int(bool(write_vectors))))
+
+    def write_bin(self, path, E=1.0, density=None, iter=0, radius=0.0, threshold=0.5, max_export_cells=0,
+                  full_field=False, json="{}"):
+        d = None if density is None else np.ascontiguousarray(density, dtype=np.float32).reshape(-1)
+        _check(lib().hexa_write_bin(self._h, str(path).encode(), _ptr(d), float(E), int(iter), float(radius),
+                                    float(threshold), int(max_export_cells), int(bool(full_field)),
+                                    json.encode()))
+
     def displacement(self):
         u = np.zeros(self.ndof)
         _check(lib().hexa_get_displacement(self._h, _ptr(u)))
@@ -382,6 +424,30 @@ class Solver:
         p = C.c_void_p()
         _check(lib().hexa_displacement_dev(self._h, C.byref(p)))
         return p.value
+
+
+def regrid_density_dev(old_grid, d_old, new_grid, lengths, hard=False, d_new=None, d_old_alpha=None,
+                       d_new_alpha=None):
+    """Density remap onto a refined grid on the device (MeshRefiner.jl:185-237).
+    Grids are (nx, ny, nz, dx, dy, dz); arrays are device pointers (ints)."""
+    og, ng = Grid(*old_grid), Grid(*new_grid)
+    _check(lib().hexa_regrid_density(C.byref(og), C.c_void_p(d_old), C.c_void_p(d_old_alpha or 0), C.byref(ng),
+                                     float(lengths[0]), float(lengths[1]), float(lengths[2]), int(bool(hard)),
+                                     C.c_void_p(d_new), C.c_void_p(d_new_alpha or 0), None))
+
+
+def regrid_density(old_grid, rho, new_grid, lengths, hard=False, alpha=None):
+    """Host-array form of regrid_density_dev; returns the new density (and the
+    new alpha when alpha is given)."""
+    og, ng = Grid(*old_grid), Grid(*new_grid)
+    r = np.ascontiguousarray(rho, dtype=np.float32).reshape(-1)
+    out = np.zeros(ng.nx * ng.ny * ng.nz, dtype=np.float32)
+    a = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float32).reshape(-1)
+    aout = None if alpha is None else np.zeros_like(out)
+    _check(lib().hexa_regrid_density_host(C.byref(og), _ptr(r), _ptr(a), C.byref(ng), float(lengths[0]),
+                                          float(lengths[1]), float(lengths[2]), int(bool(hard)), _ptr(out),
+                                          _ptr(aout)))
+    return out if alpha is None else (out, aout)
 
 
 class Server:
