@@ -71,7 +71,7 @@ enum { HEXA_METHOD_JACOBI_GPU = 0, HEXA_METHOD_GMG_GPU = 1,
 typedef struct hexa_handle hexa_handle;
 
 typedef struct {
-    int32_t nx, ny, nz;        /* elements per axis (>= 1)                */
+    int32_t nx, ny, nz;        /* elements per axis (>= 1; ny, nz <= 65534) */
     double dx, dy, dz;         /* element edge lengths (> 0)               */
 } hexa_grid;
 

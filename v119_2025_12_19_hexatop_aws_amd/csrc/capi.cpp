@@ -246,11 +246,16 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out) {
     const T omega = (T)h->opt.mg_omega;
     const T shift = (T)lv.shift;
     const int pre = std::max(1, h->opt.mg_pre), post = std::max(0, h->opt.mg_post);
-    const int swaps = (pre - 1) + post;
+    // the first sweep from x = 0 is omega D^-1 b; with a second pre-sweep the
+    // factored kernel does both in one pass (OP_SMOOTH0) without forming it
+    const bool fuse0 = pre >= 2 && h->factored && elastic_rw_supported(lv.g, std::max(sizeof(T), sizeof(DT)));
+    const int first = fuse0 ? 2 : 1;
+    const int swaps = (pre - first) + post;
     T* xa = (swaps % 2 == 0) ? out : (T*)lv.xt;
     T* xb = (swaps % 2 == 0) ? (T*)lv.xt : out;
-    launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa);
-    for (int it = 1; it < pre; ++it) {
+    if (fuse0) elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH0, fac, lv.mask, nullptr, b, dinv, omega, shift, xa);
+    else launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa);
+    for (int it = first; it < pre; ++it) {
         elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb);
         std::swap(xa, xb);
     }
@@ -687,7 +692,9 @@ int32_t hexa_device_info(char* name, int32_t len, int64_t* total_bytes, int32_t*
 int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle** out) {
     if (!grid || !out) { set_error("null argument"); return HEXA_E_INVALID; }
     *out = nullptr;
-    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || !(grid->dx > 0) || !(grid->dy > 0) || !(grid->dz > 0)) {
+    // node rows are launch-grid rows (y and z launch dimensions hold at most 65535)
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || grid->ny > 65534 || grid->nz > 65534 ||
+        !(grid->dx > 0) || !(grid->dy > 0) || !(grid->dz > 0)) {
         set_error("invalid grid %d x %d x %d (h = %g, %g, %g)", grid->nx, grid->ny, grid->nz, grid->dx,
                   grid->dy, grid->dz);
         return HEXA_E_INVALID;

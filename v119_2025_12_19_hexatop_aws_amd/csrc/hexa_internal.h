@@ -43,7 +43,9 @@ constexpr int MAX_RED_Q = 32;  // partial-sum rows per reduction buffer
 constexpr uint8_t MASK_ALL = 0x7;
 
 // Operator application modes of the fused elasticity kernel.
-enum OpMode { OP_APPLY = 0, OP_RESID = 1, OP_SMOOTH = 2 };
+// OP_SMOOTH0: one smoothing sweep started from x0 = omega D^-1 b, without
+// forming x0 (factored row-wave kernel only; u is not read).
+enum OpMode { OP_APPLY = 0, OP_RESID = 1, OP_SMOOTH = 2, OP_SMOOTH0 = 3 };
 
 // Grid description of one level.
 struct Grid3 {
@@ -108,6 +110,10 @@ template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
                       double* dot_part = nullptr);
+// Whether the row-wave kernel takes grid g (its buffer descriptors address one
+// node plane of elem_bytes-sized values with 31-bit byte counts); otherwise
+// launch_elastic_rw falls back to the tile kernel, which has no OP_SMOOTH0.
+bool elastic_rw_supported(const Grid3& g, size_t elem_bytes);
 // Upper bound on the partial sums launch_elastic_rw writes for grid g.
 int64_t elastic_rw_blocks(const Grid3& g);
 template <typename T>

@@ -158,9 +158,25 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
     const uint32_t doff = owner ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
     const int64_t vplane = 3 * plane_n;   // vector entries per node plane
 
-    auto load_own = [&](int k, T (&o)[3]) {
+    // A node plane's input.  OP_SMOOTH0 (the first smoothing sweep from
+    // x = 0) never forms its starting iterate omega D^-1 b: the plane's b and
+    // inverse diagonal are loaded and combined when the plane is consumed.
+    struct PIn {
+        T a[3], d[3];
+    };
+    const uint32_t udoff = col_in ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
+    auto load_in = [&](int k, PIn& o) {
         const bool in = (unsigned)k <= (unsigned)g.nz;
-        ld3(rsrc(u + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0), uoff, o);
+        if (MODE == OP_SMOOTH0) {
+            ld3(rsrc(b + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0), uoff, o.a);
+            ld3_dinv(rsrc(dinv + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(DT) : 0), udoff, dinv, o.d);
+        } else {
+            ld3(rsrc(u + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0), uoff, o.a);
+        }
+    };
+    auto own_value = [&](const PIn& p, T (&o)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = (MODE == OP_SMOOTH0) ? omega * p.d[c] * p.a[c] : p.a[c];
     };
     auto load_fac = [&](int lay) -> T {
         const bool in = (unsigned)lay < (unsigned)g.nz;
@@ -211,7 +227,7 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
 #pragma unroll
         for (int c = 0; c < 3; ++c) o.bv[c] = o.dv[c] = T(0);
         if (MODE != OP_APPLY) ld3(rsrc(b + vplane * ko, has_out ? vplane * (int64_t)sizeof(T) : 0), ooff, o.bv);
-        if (MODE == OP_SMOOTH)
+        if (MODE == OP_SMOOTH || MODE == OP_SMOOTH0)
             ld3_dinv(rsrc(dinv + vplane * ko, has_out ? vplane * (int64_t)sizeof(DT) : 0), doff, dinv, o.dv);
     };
     auto step = [&](const T (&Plo)[3][4], T (&Phi)[3][4], const T (&Yin)[3][4], T (&Yout)[3][4],
@@ -278,7 +294,7 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
             T v;
             if (MODE == OP_APPLY) v = Au;
             else if (MODE == OP_RESID) v = ot.bv[c] - Au;
-            else v = ulo[c] + omega * ot.dv[c] * (ot.bv[c] - Au);
+            else v = ulo[c] + omega * ot.dv[c] * (ot.bv[c] - Au);   // OP_SMOOTH, OP_SMOOTH0
             o[c] = ((ot.mb >> c) & 1) ? T(0) : v;
         }
         st3(ot.ry, ooff, o);
@@ -286,7 +302,11 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
     };
 
     T uA[3], uB[3], PA[3][4], PB[3][4], YA[3][4], YB[3][4];
-    load_own(kb - 1, uA);
+    {
+        PIn p0;
+        load_in(kb - 1, p0);
+        own_value(p0, uA);
+    }
     plane_modes(uA, PA);
     __syncthreads();   // su is rewritten by the first layer
 #pragma unroll
@@ -294,19 +314,18 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
 #pragma unroll
         for (int m = 0; m < 4; ++m) YA[c][m] = T(0);
     // two-deep prefetch ring: planes kl+1, kl+2 and factors of layers kl, kl+1
-    T r0[3], r1[3];
-    load_own(kb, r0);
-    load_own(kb + 1, r1);
+    PIn r0, r1;
+    load_in(kb, r0);
+    load_in(kb + 1, r1);
     T f0 = load_fac(kb - 1), f1 = load_fac(kb);
 
     for (int kl = kb - 1; kl < ke; kl += 2) {
         {
             Out ot;
             load_out(kl, ot);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) uB[c] = r0[c];
+            own_value(r0, uB);
             const T fl = f0;
-            load_own(kl + 3, r0);
+            load_in(kl + 3, r0);
             f0 = load_fac(kl + 2);
             step(PA, PB, YA, YB, uA, uB, fl, ot);
         }
@@ -314,10 +333,9 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         {
             Out ot;
             load_out(kl + 1, ot);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) uA[c] = r1[c];
+            own_value(r1, uA);
             const T fl = f1;
-            load_own(kl + 4, r1);
+            load_in(kl + 4, r1);
             f1 = load_fac(kl + 3);
             step(PB, PA, YB, YA, uB, uA, fl, ot);
         }
@@ -358,6 +376,10 @@ RwShape rw_shape(const Grid3& g) {
 }
 }  // namespace
 
+bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
+    return 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
+}
+
 int64_t elastic_rw_blocks(const Grid3& g) {
     const RwShape r = rw_shape(g);
     return (int64_t)r.tiles_x * r.tiles_y * r.tiles_z;
@@ -368,7 +390,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
                       double* dot_part) {
     // buffer descriptors address one node plane (byte counts are 31-bit)
-    if (3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)std::max(sizeof(T), sizeof(DT)) > INT32_MAX) {
+    if (!elastic_rw_supported(g, std::max(sizeof(T), sizeof(DT)))) {
         launch_elastic_fact<T, DT>(s, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
         return 0;
     }
@@ -383,6 +405,10 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
         case OP_RESID:
             k_elastic_rw<T, DT, OP_RESID><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
+            break;
+        case OP_SMOOTH0:
+            k_elastic_rw<T, DT, OP_SMOOTH0><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
+                                                                   r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
             break;
         default:
             k_elastic_rw<T, DT, OP_SMOOTH><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,

@@ -19,75 +19,102 @@ inline unsigned grid_for(int64_t n, int threads = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Transfers run one node row per block row: threads along x, blockIdx.y /
+// blockIdx.z = the row's y / z index, so no index division at all.
+constexpr int TB = 128;
+
 // r_c(I,J,K) = sum over the 3x3x3 fine nodes around (2I,2J,2K) of
 // w(di) w(dj) w(dk) r_f, w(0) = 1, w(+-1) = 1/2; held coarse DOFs -> 0.
 template <typename T>
-__global__ void k_restrict(Grid3 f, Grid3 c, const T* __restrict__ rf, const uint8_t* __restrict__ cmask,
-                           T* __restrict__ rc) {
+__global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, const T* __restrict__ rf,
+                                                 const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
     const int CNX = c.nx + 1, CNY = c.ny + 1;
     const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
-    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < c.nn;
-         n += (int64_t)gridDim.x * blockDim.x) {
-        const int I = (int)(n % CNX), J = (int)((n / CNX) % CNY), K = (int)(n / ((int64_t)CNX * CNY));
-        const uint8_t mb = cmask[n];
-        T s0 = 0, s1 = 0, s2 = 0;
-        for (int dk = -1; dk <= 1; ++dk) {
-            const int k = 2 * K + dk;
-            if ((unsigned)k >= (unsigned)FNZ) continue;
-            const T wk = dk == 0 ? T(1) : T(0.5);
-            for (int dj = -1; dj <= 1; ++dj) {
-                const int j = 2 * J + dj;
-                if ((unsigned)j >= (unsigned)FNY) continue;
-                const T wjk = wk * (dj == 0 ? T(1) : T(0.5));
-                for (int di = -1; di <= 1; ++di) {
-                    const int i = 2 * I + di;
-                    if ((unsigned)i >= (unsigned)FNX) continue;
-                    const T w = wjk * (di == 0 ? T(1) : T(0.5));
-                    const int64_t m = i + (int64_t)FNX * (j + (int64_t)FNY * k);
-                    s0 += w * rf[3 * m];
-                    s1 += w * rf[3 * m + 1];
-                    s2 += w * rf[3 * m + 2];
-                }
+    const int I = blockIdx.x * TB + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
+    if (I >= CNX) return;
+    const int64_t n = I + (int64_t)CNX * (J + (int64_t)CNY * K);
+    const uint8_t mb = cmask[n];
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int dk = -1; dk <= 1; ++dk) {
+        const int k = 2 * K + dk;
+        if ((unsigned)k >= (unsigned)FNZ) continue;
+        const T wk = dk == 0 ? T(1) : T(0.5);
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int j = 2 * J + dj;
+            if ((unsigned)j >= (unsigned)FNY) continue;
+            const T wjk = wk * (dj == 0 ? T(1) : T(0.5));
+            const T* row = rf + 3 * (int64_t)FNX * (j + (int64_t)FNY * k);
+#pragma unroll
+            for (int di = -1; di <= 1; ++di) {
+                const int i = 2 * I + di;
+                if ((unsigned)i >= (unsigned)FNX) continue;
+                const T w = wjk * (di == 0 ? T(1) : T(0.5));
+                s0 += w * row[3 * i];
+                s1 += w * row[3 * i + 1];
+                s2 += w * row[3 * i + 2];
             }
         }
-        rc[3 * n] = (mb & 1) ? T(0) : s0;
-        rc[3 * n + 1] = (mb & 2) ? T(0) : s1;
-        rc[3 * n + 2] = (mb & 4) ? T(0) : s2;
     }
+    rc[3 * n] = (mb & 1) ? T(0) : s0;
+    rc[3 * n + 1] = (mb & 2) ? T(0) : s1;
+    rc[3 * n + 2] = (mb & 4) ? T(0) : s2;
 }
 
-// x_f += P x_c, masked by the fine supports.  Along each axis an even fine
-// index coincides with coarse node i/2 (weight 1); an odd one lies midway
-// between (i-1)/2 and (i+1)/2 (weights 1/2).
+// x_f += P x_c, masked by the fine supports.  One thread per coarse cell
+// (I,J,K) updates the up-to-8 fine nodes (2I + a, 2J + b, 2K + c), a,b,c in
+// {0,1}, from the 8 coarse nodes at the cell's corners: along each axis an
+// even fine index coincides with a coarse node (weight 1), an odd one lies
+// midway (weights 1/2).  Coarse nodes beyond the coarse grid (the last fine
+// node of an odd axis under a floor(n/2) coarse grid) contribute nothing.
 template <typename T>
-__global__ void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
-                              const uint8_t* __restrict__ fmask, T* __restrict__ xf) {
-    const int CNX = c.nx + 1, CNY = c.ny + 1;
-    const int FNX = f.nx + 1, FNY = f.ny + 1;
-    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < f.nn;
-         n += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(n % FNX), j = (int)((n / FNX) % FNY), k = (int)(n / ((int64_t)FNX * FNY));
-        const uint8_t mb = fmask[n];
-        if ((mb & 7) == 7) continue;
-        const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-        const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-        const T wi = ni == 2 ? T(0.5) : T(1), wj = nj == 2 ? T(0.5) : T(1), wk = nk == 2 ? T(0.5) : T(1);
-        T s0 = 0, s1 = 0, s2 = 0;
-        for (int a = 0; a < nk; ++a)
-            for (int bb = 0; bb < nj; ++bb)
-                for (int cc = 0; cc < ni; ++cc) {
-                    // a coarse grid of floor(n/2) elements does not cover the
-                    // last fine node of an odd axis: its far weight is dropped
-                    if (i0 + cc >= CNX || j0 + bb >= CNY || k0 + a >= c.nz + 1) continue;
-                    const int64_t m = (i0 + cc) + (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
-                    s0 += xc[3 * m];
-                    s1 += xc[3 * m + 1];
-                    s2 += xc[3 * m + 2];
-                }
-        const T w = wi * wj * wk;
-        if (!(mb & 1)) xf[3 * n] += w * s0;
-        if (!(mb & 2)) xf[3 * n + 1] += w * s1;
-        if (!(mb & 4)) xf[3 * n + 2] += w * s2;
+__global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
+                                                    const uint8_t* __restrict__ fmask, T* __restrict__ xf) {
+    const int CNX = c.nx + 1, CNY = c.ny + 1, CNZ = c.nz + 1;
+    const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
+    const int I = blockIdx.x * TB + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
+    if (I >= CNX || 2 * I >= FNX || 2 * J >= FNY || 2 * K >= FNZ) return;
+    T v[2][2][2][3];
+#pragma unroll
+    for (int dc = 0; dc < 2; ++dc)
+#pragma unroll
+        for (int db = 0; db < 2; ++db)
+#pragma unroll
+            for (int da = 0; da < 2; ++da) {
+                const bool in = I + da < CNX && J + db < CNY && K + dc < CNZ;
+                const int64_t m = in ? (I + da) + (int64_t)CNX * ((J + db) + (int64_t)CNY * (K + dc)) : 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) v[dc][db][da][q] = in ? xc[3 * m + q] : T(0);
+            }
+#pragma unroll
+    for (int c2 = 0; c2 < 2; ++c2) {
+        const int k = 2 * K + c2;
+        if (k >= FNZ) continue;
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) {
+            const int j = 2 * J + b2;
+            if (j >= FNY) continue;
+#pragma unroll
+            for (int a2 = 0; a2 < 2; ++a2) {
+                const int i = 2 * I + a2;
+                if (i >= FNX) continue;
+                const int64_t n = i + (int64_t)FNX * (j + (int64_t)FNY * k);
+                const uint8_t mb = fmask[n];
+                T s[3] = {T(0), T(0), T(0)};
+                // the corners this node averages: offsets 0 only (even) or 0 and 1 (odd)
+#pragma unroll
+                for (int dc = 0; dc <= c2; ++dc)
+#pragma unroll
+                    for (int db = 0; db <= b2; ++db)
+#pragma unroll
+                        for (int da = 0; da <= a2; ++da)
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) s[q] += v[dc][db][da][q];
+                const T w = (a2 ? T(0.5) : T(1)) * (b2 ? T(0.5) : T(1)) * (c2 ? T(0.5) : T(1));
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    if (!((mb >> q) & 1)) xf[3 * n + q] += w * s[q];
+            }
+        }
     }
 }
 
@@ -218,13 +245,15 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
 template <typename T>
 void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
                      const uint8_t* cmask, T* rc) {
-    k_restrict<T><<<grid_for(coarse.nn), 256, 0, s>>>(fine, coarse, rf, cmask, rc);
+    const dim3 grid((unsigned)((coarse.nx + 1 + TB - 1) / TB), (unsigned)(coarse.ny + 1), (unsigned)(coarse.nz + 1));
+    k_restrict<T><<<grid, TB, 0, s>>>(fine, coarse, rf, cmask, rc);
 }
 
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf) {
-    k_prolong_add<T><<<grid_for(fine.nn), 256, 0, s>>>(fine, coarse, xc, fmask, xf);
+    const dim3 grid((unsigned)((coarse.nx + 1 + TB - 1) / TB), (unsigned)(coarse.ny + 1), (unsigned)(coarse.nz + 1));
+    k_prolong_add<T><<<grid, TB, 0, s>>>(fine, coarse, xc, fmask, xf);
 }
 
 void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
