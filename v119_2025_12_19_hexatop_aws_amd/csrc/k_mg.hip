@@ -8,6 +8,8 @@
 // the 2x tensor-product stencil (weights 1, 1/2, 1/4, 1/8), so both P and
 // R = P^T are written here as gathers: no coordinates, no atomics, and R is
 // the exact adjoint of P.
+#include <algorithm>
+
 #include "hexa_internal.h"
 
 namespace hexa {
@@ -19,9 +21,20 @@ inline unsigned grid_for(int64_t n, int threads = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-// Transfers run one node row per block row: threads along x, blockIdx.y /
-// blockIdx.z = the row's y / z index, so no index division at all.
+// Transfers run node rows: threads along x, each block row walks a strided
+// set of (y, z) rows, so blocks live long enough to keep the memory busy.
 constexpr int TB = 128;
+
+// A node's three components, loaded and stored as one vector access (12 or
+// 24 contiguous bytes per lane, lanes contiguous along x).
+template <typename T>
+struct alignas(sizeof(T)) V3 {
+    T x, y, z;
+};
+template <typename T>
+__device__ __forceinline__ V3<T> ld(const T* p, int64_t node) { return *reinterpret_cast<const V3<T>*>(p + 3 * node); }
+template <typename T>
+__device__ __forceinline__ void st(T* p, int64_t node, const V3<T>& v) { *reinterpret_cast<V3<T>*>(p + 3 * node) = v; }
 
 // r_c(I,J,K) = sum over the 3x3x3 fine nodes around (2I,2J,2K) of
 // w(di) w(dj) w(dk) r_f, w(0) = 1, w(+-1) = 1/2; held coarse DOFs -> 0.
@@ -30,11 +43,14 @@ __global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, const T* __re
                                                  const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
     const int CNX = c.nx + 1, CNY = c.ny + 1;
     const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
-    const int I = blockIdx.x * TB + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
+    const int I = blockIdx.x * TB + threadIdx.x;
     if (I >= CNX) return;
+    const int rows = CNY * (c.nz + 1);
+    for (int rr = blockIdx.y; rr < rows; rr += gridDim.y) {
+    const int J = rr % CNY, K = rr / CNY;
     const int64_t n = I + (int64_t)CNX * (J + (int64_t)CNY * K);
     const uint8_t mb = cmask[n];
-    T s0 = 0, s1 = 0, s2 = 0;
+    V3<T> s{T(0), T(0), T(0)};
     for (int dk = -1; dk <= 1; ++dk) {
         const int k = 2 * K + dk;
         if ((unsigned)k >= (unsigned)FNZ) continue;
@@ -43,78 +59,64 @@ __global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, const T* __re
             const int j = 2 * J + dj;
             if ((unsigned)j >= (unsigned)FNY) continue;
             const T wjk = wk * (dj == 0 ? T(1) : T(0.5));
-            const T* row = rf + 3 * (int64_t)FNX * (j + (int64_t)FNY * k);
+            const int64_t row = (int64_t)FNX * (j + (int64_t)FNY * k);
 #pragma unroll
             for (int di = -1; di <= 1; ++di) {
                 const int i = 2 * I + di;
                 if ((unsigned)i >= (unsigned)FNX) continue;
                 const T w = wjk * (di == 0 ? T(1) : T(0.5));
-                s0 += w * row[3 * i];
-                s1 += w * row[3 * i + 1];
-                s2 += w * row[3 * i + 2];
+                const V3<T> v = ld(rf, row + i);
+                s.x += w * v.x;
+                s.y += w * v.y;
+                s.z += w * v.z;
             }
         }
     }
-    rc[3 * n] = (mb & 1) ? T(0) : s0;
-    rc[3 * n + 1] = (mb & 2) ? T(0) : s1;
-    rc[3 * n + 2] = (mb & 4) ? T(0) : s2;
+    st(rc, n, V3<T>{(mb & 1) ? T(0) : s.x, (mb & 2) ? T(0) : s.y, (mb & 4) ? T(0) : s.z});
+    }
 }
 
-// x_f += P x_c, masked by the fine supports.  One thread per coarse cell
-// (I,J,K) updates the up-to-8 fine nodes (2I + a, 2J + b, 2K + c), a,b,c in
-// {0,1}, from the 8 coarse nodes at the cell's corners: along each axis an
-// even fine index coincides with a coarse node (weight 1), an odd one lies
-// midway (weights 1/2).  Coarse nodes beyond the coarse grid (the last fine
+// x_f += P x_c, masked by the fine supports; one thread per fine node, node
+// rows along the block.  Along each axis an even fine index coincides with
+// coarse node i/2 (weight 1), an odd one lies midway between (i-1)/2 and
+// (i+1)/2 (weights 1/2).  Coarse nodes beyond the coarse grid (the last fine
 // node of an odd axis under a floor(n/2) coarse grid) contribute nothing.
 template <typename T>
 __global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
                                                     const uint8_t* __restrict__ fmask, T* __restrict__ xf) {
     const int CNX = c.nx + 1, CNY = c.ny + 1, CNZ = c.nz + 1;
-    const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
-    const int I = blockIdx.x * TB + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
-    if (I >= CNX || 2 * I >= FNX || 2 * J >= FNY || 2 * K >= FNZ) return;
-    T v[2][2][2][3];
-#pragma unroll
-    for (int dc = 0; dc < 2; ++dc)
-#pragma unroll
-        for (int db = 0; db < 2; ++db)
-#pragma unroll
-            for (int da = 0; da < 2; ++da) {
-                const bool in = I + da < CNX && J + db < CNY && K + dc < CNZ;
-                const int64_t m = in ? (I + da) + (int64_t)CNX * ((J + db) + (int64_t)CNY * (K + dc)) : 0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) v[dc][db][da][q] = in ? xc[3 * m + q] : T(0);
-            }
-#pragma unroll
-    for (int c2 = 0; c2 < 2; ++c2) {
-        const int k = 2 * K + c2;
-        if (k >= FNZ) continue;
-#pragma unroll
-        for (int b2 = 0; b2 < 2; ++b2) {
-            const int j = 2 * J + b2;
-            if (j >= FNY) continue;
-#pragma unroll
-            for (int a2 = 0; a2 < 2; ++a2) {
-                const int i = 2 * I + a2;
-                if (i >= FNX) continue;
-                const int64_t n = i + (int64_t)FNX * (j + (int64_t)FNY * k);
-                const uint8_t mb = fmask[n];
-                T s[3] = {T(0), T(0), T(0)};
-                // the corners this node averages: offsets 0 only (even) or 0 and 1 (odd)
-#pragma unroll
-                for (int dc = 0; dc <= c2; ++dc)
-#pragma unroll
-                    for (int db = 0; db <= b2; ++db)
-#pragma unroll
-                        for (int da = 0; da <= a2; ++da)
-#pragma unroll
-                            for (int q = 0; q < 3; ++q) s[q] += v[dc][db][da][q];
-                const T w = (a2 ? T(0.5) : T(1)) * (b2 ? T(0.5) : T(1)) * (c2 ? T(0.5) : T(1));
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    if (!((mb >> q) & 1)) xf[3 * n + q] += w * s[q];
+    const int FNX = f.nx + 1, FNY = f.ny + 1;
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= FNX) return;
+    const int rows = FNY * (f.nz + 1);
+    for (int rr = blockIdx.y; rr < rows; rr += gridDim.y) {
+    const int j = rr % FNY, k = rr / FNY;
+    const int64_t n = i + (int64_t)FNX * (j + (int64_t)FNY * k);
+    const uint8_t mb = fmask[n];
+    if ((mb & 7) == 7) continue;
+    const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    V3<T> s{T(0), T(0), T(0)};
+    for (int a = 0; a < nk; ++a) {
+        if (k0 + a >= CNZ) continue;
+        for (int bb = 0; bb < nj; ++bb) {
+            if (j0 + bb >= CNY) continue;
+            const int64_t row = (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
+            for (int cc = 0; cc < ni; ++cc) {
+                if (i0 + cc >= CNX) continue;
+                const V3<T> v = ld(xc, row + i0 + cc);
+                s.x += v.x;
+                s.y += v.y;
+                s.z += v.z;
             }
         }
+    }
+    const T w = (ni == 2 ? T(0.5) : T(1)) * (nj == 2 ? T(0.5) : T(1)) * (nk == 2 ? T(0.5) : T(1));
+    V3<T> x = ld(xf, n);
+    if (!(mb & 1)) x.x += w * s.x;
+    if (!(mb & 2)) x.y += w * s.y;
+    if (!(mb & 4)) x.z += w * s.z;
+    st(xf, n, x);
     }
 }
 
@@ -245,14 +247,16 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
 template <typename T>
 void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
                      const uint8_t* cmask, T* rc) {
-    const dim3 grid((unsigned)((coarse.nx + 1 + TB - 1) / TB), (unsigned)(coarse.ny + 1), (unsigned)(coarse.nz + 1));
+    const int64_t rows = (int64_t)(coarse.ny + 1) * (coarse.nz + 1);
+    const dim3 grid((unsigned)((coarse.nx + 1 + TB - 1) / TB), (unsigned)std::min<int64_t>(rows, 4096));
     k_restrict<T><<<grid, TB, 0, s>>>(fine, coarse, rf, cmask, rc);
 }
 
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf) {
-    const dim3 grid((unsigned)((coarse.nx + 1 + TB - 1) / TB), (unsigned)(coarse.ny + 1), (unsigned)(coarse.nz + 1));
+    const int64_t rows = (int64_t)(fine.ny + 1) * (fine.nz + 1);
+    const dim3 grid((unsigned)((fine.nx + 1 + TB - 1) / TB), (unsigned)std::min<int64_t>(rows, 4096));
     k_prolong_add<T><<<grid, TB, 0, s>>>(fine, coarse, xc, fmask, xf);
 }
 
