@@ -8,9 +8,13 @@
 //    and the element column to its +x side; neighbouring node values and the
 //    +x corner results move between lanes with DPP wave shifts, not LDS.
 //    Lanes 0 and 63 are halo: 62 owned columns per 64 loaded.
-//  * y runs across the BY waves of a block; the +y node row and the +y
-//    corner results are exchanged through LDS (3 values each per node and
-//    layer).  Waves 0 and BY-1 are halo rows; the last wave only loads.
+//  * y runs across the BY waves of a block: wave w owns element row w and
+//    node row w (BY+1 node rows per tile, rows 0 and BY are halo).  The +y
+//    node row and the +y corner results are exchanged through LDS (3 values
+//    each per node and layer); the top node row is loaded by waves 0-2, one
+//    component each.  Both exchanges of a layer share one barrier: the node
+//    row staged is the one two planes ahead, and the buffers alternate with
+//    the layer parity.
 //  * z is marched, with the node planes and element factors of the next two
 //    layers in flight.  The butterfly is separable, so the x-y part of a
 //    node plane's transform is computed once and used by the element layers
@@ -26,6 +30,7 @@
 // result on every run.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "hexa_internal.h"
 
@@ -38,7 +43,7 @@ namespace {
 constexpr int WL = 64;           // lanes per wavefront = node columns per tile
 constexpr int OWN_X = WL - 2;    // owned node columns per tile
 constexpr int BY = HEXA_RW_BY;   // wavefronts per block = node rows per tile
-constexpr int OWN_Y = BY - 2;    // owned node rows per tile
+constexpr int OWN_Y = BY - 1;    // owned node rows per tile
 constexpr int NXCD = 8;
 
 // DPP wave shifts (GFX9 encodings): lane l reads lane l+1 / lane l-1; the
@@ -120,6 +125,29 @@ __device__ __forceinline__ void ld3_dinv(Rsrc r, uint32_t off, const uint16_t*, 
         o[c] = (T)__uint_as_float(((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off + 2 * c, 0, 0)) << 16);
 }
 
+// A wave-uniform coefficient held in a VGPR.  The FP64 kernel's scalar
+// registers cannot hold all of its coefficients; the coupling coefficients
+// live in vector registers instead of being spilled and re-read per use.
+__device__ __forceinline__ double in_vgpr(double x) {
+    double r;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+__device__ __forceinline__ float in_vgpr(float x) { return x; }
+
+template <typename T>
+__device__ __forceinline__ T ld1_dinv(Rsrc r, uint32_t off, const double*) {
+    return (T)ld1(r, off, 0.0);
+}
+template <typename T>
+__device__ __forceinline__ T ld1_dinv(Rsrc r, uint32_t off, const float*) {
+    return (T)ld1(r, off, 0.0f);
+}
+template <typename T>
+__device__ __forceinline__ T ld1_dinv(Rsrc r, uint32_t off, const uint16_t*) {
+    return (T)__uint_as_float(((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)) << 16);
+}
+
 template <typename T, typename DT, int MODE>
 __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef, const T* __restrict__ fac,
                                                        const uint8_t* __restrict__ mask,
@@ -127,10 +155,19 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
                                                        const DT* __restrict__ dinv, T omega, T shift,
                                                        T* __restrict__ y, int kchunk, int tiles_x, int tiles_y,
                                                        int tiles, double* __restrict__ dot_part) {
-    __shared__ T su[BY][3][WL];   // node values of each wave's row, current plane
-    __shared__ T sc[BY][3][WL];   // each wave's results for the row above it
+    // double-buffered by layer parity, so one barrier per layer suffices
+    __shared__ T su[2][BY + 1][3][WL];   // node values of each wave's row, plane kl+2
+    __shared__ T sc[2][BY][3][WL];   // each wave's results for the row above it
 
     const int l = threadIdx.x, w = threadIdx.y;
+    T cl[3][2], cm[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cl[d][q] = in_vgpr(coef.cl[d][q]);
+            cm[d][q] = in_vgpr(coef.cm[d][q]);
+        }
     // XCD-contiguous tile order: workgroup L runs on XCD L % 8
     const int L = blockIdx.x;
     const int xcd = L % NXCD, q = tiles / NXCD, r = tiles % NXCD;
@@ -141,11 +178,9 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
     const int j = ty * OWN_Y - 1 + w;
     const int kb = tz * kchunk;
     const int ke = min(kb + kchunk, g.nz + 1);
-    // the last wave only supplies its node row to the wave below
-    const bool compute = w < BY - 1;
     const bool col_in = (unsigned)i <= (unsigned)g.nx && (unsigned)j <= (unsigned)g.ny;
-    const bool elem = l < WL - 1 && compute && (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-    const bool owner = l >= 1 && l <= WL - 2 && w >= 1 && w <= BY - 2 && col_in;
+    const bool elem = l < WL - 1 && (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const bool owner = l >= 1 && l <= WL - 2 && w >= 1 && col_in;
     const int64_t plane_n = (int64_t)(g.nx + 1) * (g.ny + 1);
     const int64_t plane_e = (int64_t)g.nx * g.ny;
 
@@ -157,54 +192,73 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
     const uint32_t moff = owner ? (uint32_t)nrow : OOB;
     const uint32_t doff = owner ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
     const int64_t vplane = 3 * plane_n;   // vector entries per node plane
+    // the tile's top node row (BY waves own element rows 0..BY-1 and node
+    // rows 1..BY-1): waves 0-2 load component w of its nodes as an extra
+    const int jx = ty * OWN_Y - 1 + BY;
+    const bool x_in = w < 3 && (unsigned)i <= (unsigned)g.nx && (unsigned)jx <= (unsigned)g.ny;
+    const int64_t xent = (i + (int64_t)(g.nx + 1) * jx) * 3 + w;
+    const uint32_t xoff = x_in ? (uint32_t)(xent * sizeof(T)) : OOB;
+    const uint32_t xdoff = x_in ? (uint32_t)(xent * sizeof(DT)) : OOB;
 
     // A node plane's input.  OP_SMOOTH0 (the first smoothing sweep from
     // x = 0) never forms its starting iterate omega D^-1 b: the plane's b and
     // inverse diagonal are loaded and combined when the plane is consumed.
     struct PIn {
-        T a[3], d[3];
+        T a[3], d[3], xa, xd;
     };
     const uint32_t udoff = col_in ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
     auto load_in = [&](int k, PIn& o) {
         const bool in = (unsigned)k <= (unsigned)g.nz;
         if (MODE == OP_SMOOTH0) {
-            ld3(rsrc(b + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0), uoff, o.a);
-            ld3_dinv(rsrc(dinv + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(DT) : 0), udoff, dinv, o.d);
+            const Rsrc rb = rsrc(b + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0);
+            const Rsrc rd = rsrc(dinv + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(DT) : 0);
+            ld3(rb, uoff, o.a);
+            ld3_dinv(rd, udoff, dinv, o.d);
+            o.xa = ld1(rb, xoff, T(0));
+            o.xd = ld1_dinv<T>(rd, xdoff, dinv);
         } else {
-            ld3(rsrc(u + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0), uoff, o.a);
+            const Rsrc ru = rsrc(u + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0);
+            ld3(ru, uoff, o.a);
+            o.xa = ld1(ru, xoff, T(0));
         }
     };
-    auto own_value = [&](const PIn& p, T (&o)[3]) {
+    // own node values of a plane, and the extra top-row value
+    auto own_value = [&](const PIn& p, T (&o)[3], T& x) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) o[c] = (MODE == OP_SMOOTH0) ? omega * p.d[c] * p.a[c] : p.a[c];
+        x = (MODE == OP_SMOOTH0) ? omega * p.xd * p.xa : p.xa;
+    };
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    // stage a plane's own row (and the top row) for the wave below
+    auto stage = [&](auto par, const T (&o)[3], const T x) {
+        constexpr int P = decltype(par)::value;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) su[P][w][c][l] = o[c];
+        if (w < 3) su[P][BY][w][l] = x;
     };
     auto load_fac = [&](int lay) -> T {
         const bool in = (unsigned)lay < (unsigned)g.nz;
         return ld1(rsrc(fac + (in ? plane_e * lay : 0), in ? plane_e * (int64_t)sizeof(T) : 0), foff, T(0));
     };
     // x-y modes of the element column's 4 corners in one node plane, from
-    // this lane's node value: P[c][mx + 2 my].  Contains a barrier.
-    auto plane_modes = [&](const T (&own)[3], T (&P)[3][4]) {
+    // this lane's node value and the value of the node above it (row w+1):
+    // P[c][mx + 2 my].
+    auto plane_modes = [&](const T (&own)[3], const T (&up)[3], T (&P)[3][4]) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) su[w][c][l] = own[c];
-        __syncthreads();
-        if (compute) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const T up = su[w + 1][c][l];
-                const T a = own[c], bx = from_next(own[c]), cy = up, d = from_next(up);
-                const T s0 = a + bx, d0 = bx - a, s1 = cy + d, d1 = d - cy;
-                P[c][0] = s0 + s1;
-                P[c][1] = d0 + d1;
-                P[c][2] = s1 - s0;
-                P[c][3] = d1 - d0;
-            }
+        for (int c = 0; c < 3; ++c) {
+            const T a = own[c], bx = from_next(own[c]), cy = up[c], d = from_next(up[c]);
+            const T s0 = a + bx, d0 = bx - a, s1 = cy + d, d1 = d - cy;
+            P[c][0] = s0 + s1;
+            P[c][1] = d0 + d1;
+            P[c][2] = s1 - s0;
+            P[c][3] = d1 - d0;
         }
     };
 
     // One element layer kl (node planes kl, kl+1) and the output of node
     // plane kl.  Plo/Yin: x-y modes of plane kl and the layer below's modal
-    // results for plane kl; Phi/Yout: the same for plane kl+1.
+    // results for plane kl; Yout: the modal results for plane kl+1.
     struct Out {
         Rsrc ry;
         uint32_t mb;
@@ -230,11 +284,18 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         if (MODE == OP_SMOOTH || MODE == OP_SMOOTH0)
             ld3_dinv(rsrc(dinv + vplane * ko, has_out ? vplane * (int64_t)sizeof(DT) : 0), doff, dinv, o.dv);
     };
-    auto step = [&](const T (&Plo)[3][4], T (&Phi)[3][4], const T (&Yin)[3][4], T (&Yout)[3][4],
-                    const T (&ulo)[3], const T (&uhi)[3], const T fl, const Out& ot) {
-        plane_modes(uhi, Phi);   // barrier A
-        T N0[3] = {T(0), T(0), T(0)};
-        if (compute) {
+    // Layer kl.  In: Plo, Yin (plane kl), ulo, uhi (own values of planes kl,
+    // kl+1), uphi (row w+1 of plane kl+1), unx (own values of plane kl+2).
+    // Out: Phi, Yout (plane kl+1), upnx (row w+1 of plane kl+2).  The +y
+    // exchange of this layer's results and the staging of plane kl+2's row
+    // share one barrier; buffers alternate with the layer parity PAR.
+    auto step = [&](auto par, const T (&Plo)[3][4], T (&Phi)[3][4], const T (&Yin)[3][4], T (&Yout)[3][4],
+                    const T (&ulo)[3], const T (&uhi)[3], const T (&uphi)[3], const T (&unx)[3], const T xnx,
+                    T (&upnx)[3], const T fl, const Out& ot) {
+        constexpr int P = decltype(par)::value;
+        T N0[3];
+        {
+            plane_modes(uhi, uphi, Phi);
             // modes v[c][m], m = mx + 2my + 4mz (mode 0 is not used)
             T v[3][8];
 #pragma unroll
@@ -258,8 +319,8 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
 #pragma unroll
                     for (int qq = 0; qq < 2; ++qq) {
                         const int mq = qq << d;
-                        yv[(1 << c) | mq] += coef.cl[d][qq] * v[bb][(1 << bb) | mq];
-                        yv[(1 << bb) | mq] += coef.cm[d][qq] * v[bb][(1 << c) | mq];
+                        yv[(1 << c) | mq] += cl[d][qq] * v[bb][(1 << bb) | mq];
+                        yv[(1 << bb) | mq] += cm[d][qq] * v[bb][(1 << c) | mq];
                     }
                 }
                 // element factor and the z part of the transposed butterfly:
@@ -283,13 +344,15 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
                 N1[c] = c01 + from_prev(c11);
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) sc[w][c][l] = N1[c];
+            for (int c = 0; c < 3; ++c) sc[P][w][c][l] = N1[c];
         }
-        __syncthreads();   // barrier B
+        stage(par, unx, xnx);
+        __syncthreads();
         T o[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T below = sc[w > 0 ? w - 1 : 0][c][l];
+            upnx[c] = su[P][w + 1][c][l];
+            const T below = sc[P][w > 0 ? w - 1 : 0][c][l];
             const T Au = N0[c] + (owner ? below : T(0)) + shift * ulo[c];
             T v;
             if (MODE == OP_APPLY) v = Au;
@@ -301,43 +364,65 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         if (dot_part && ot.has_out && owner) acc += (double)ulo[0] * o[0] + (double)ulo[1] * o[1] + (double)ulo[2] * o[2];
     };
 
-    T uA[3], uB[3], PA[3][4], PB[3][4], YA[3][4], YB[3][4];
+    // planes kb-1 and kb: own values, staged rows, modes of plane kb-1
+    T u0[3], u1[3], u2[3], up1[3], up2[3], PA[3][4], PB[3][4], YA[3][4], YB[3][4], xnx;
     {
-        PIn p0;
+        PIn p0, p1;
         load_in(kb - 1, p0);
-        own_value(p0, uA);
+        load_in(kb, p1);
+        own_value(p0, u0, xnx);
+        stage(P0{}, u0, xnx);
+        own_value(p1, u1, xnx);
+        stage(P1{}, u1, xnx);
+        __syncthreads();
+        T up0[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            up0[c] = su[0][w + 1][c][l];
+            up1[c] = su[1][w + 1][c][l];
+        }
+        plane_modes(u0, up0, PA);
+        __syncthreads();   // the staging buffers are rewritten by the first layer
     }
-    plane_modes(uA, PA);
-    __syncthreads();   // su is rewritten by the first layer
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int m = 0; m < 4; ++m) YA[c][m] = T(0);
-    // two-deep prefetch ring: planes kl+1, kl+2 and factors of layers kl, kl+1
+    // two-deep prefetch ring: planes kl+2, kl+3 and factors of layers kl, kl+1
     PIn r0, r1;
-    load_in(kb, r0);
-    load_in(kb + 1, r1);
+    load_in(kb + 1, r0);
+    load_in(kb + 2, r1);
     T f0 = load_fac(kb - 1), f1 = load_fac(kb);
 
+    // own values rotate u0 -> u1 -> u2 (planes kl, kl+1, kl+2), and the
+    // staged rows up1 -> up2, with the layer parity
     for (int kl = kb - 1; kl < ke; kl += 2) {
         {
             Out ot;
             load_out(kl, ot);
-            own_value(r0, uB);
+            own_value(r0, u2, xnx);
             const T fl = f0;
-            load_in(kl + 3, r0);
+            load_in(kl + 4, r0);
             f0 = load_fac(kl + 2);
-            step(PA, PB, YA, YB, uA, uB, fl, ot);
+            step(P0{}, PA, PB, YA, YB, u0, u1, up1, u2, xnx, up2, fl, ot);
         }
         if (kl + 1 >= ke) break;
         {
             Out ot;
             load_out(kl + 1, ot);
-            own_value(r1, uA);
+            own_value(r1, u0, xnx);
             const T fl = f1;
-            load_in(kl + 4, r1);
+            load_in(kl + 5, r1);
             f1 = load_fac(kl + 3);
-            step(PB, PA, YB, YA, uB, uA, fl, ot);
+            step(P1{}, PB, PA, YB, YA, u1, u2, up2, u0, xnx, up1, fl, ot);
+        }
+        // planes kl+2, kl+3 are now in (u2, up2... u0) order: rotate names
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T t = u1[c];
+            u1[c] = u0[c];
+            u0[c] = u2[c];
+            u2[c] = t;
         }
     }
     if (dot_part) {
