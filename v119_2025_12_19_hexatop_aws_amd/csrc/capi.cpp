@@ -377,6 +377,9 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
     const bool mx = use_mg && mixed(h) && h->levels.size() > 1;
     float* r32 = mx ? (float*)h->levels[0].b : nullptr;
     const float* z32 = mx ? (const float*)h->levels[0].x : nullptr;
+    // there, with CPU semantics (no best-iterate copy of x after the check),
+    // x += alpha p moves from the update to the p update, which reads p anyway
+    const bool defer_x = mx && cpu;
     for (int k = 1; k <= prm->max_iter && !stop; ++k) {
         // Ap = (K + sI) p, with the per-block partials of (p, Ap)
         const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask,
@@ -388,8 +391,8 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
             launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
             launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
         }
-        launch_pcg_update(st, nd, h->d_state, h->d_x, h->d_p, h->d_r, h->d_Ap, pk == 0 ? h->d_minv : nullptr,
-                          h->d_partials, r32);
+        launch_pcg_update(st, nd, h->d_state, defer_x ? nullptr : h->d_x, h->d_p, h->d_r, h->d_Ap,
+                          pk == 0 ? h->d_minv : nullptr, h->d_partials, r32, k);
         const int check = cpu ? 1 : ((k % check_every == 0) || k == 1);
         if (pk == 2) {
             // z = M r and (r, z) for the standard beta (GPUSolver.jl:409-417)
@@ -402,7 +405,7 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
             vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32);
             launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_partials);
             launch_pcg_flex_beta(st, h->d_state, h->d_partials);
-            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32);
+            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, defer_x ? h->d_x : nullptr, k);
         } else if (use_mg) {
             apply_mg(h, h->d_r, h->d_z);
             launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);

@@ -57,6 +57,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_init_state(PcgState* st, const 
         st->halted = PCG_RUNNING;
         st->improved = 0;
         st->last_k = 0;
+        st->x_iter = 0;
     }
 }
 
@@ -127,25 +128,54 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const dou
 
 // x += alpha p; r -= alpha Ap; partials: (r,r) and, when minv is given,
 // (r, minv r) for the Jacobi preconditioner.  r32, when given, receives the
-// new residual in FP32 (the mixed V-cycle's input).
-__global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, const PcgState* st, double* __restrict__ x,
+// new residual in FP32 (the mixed V-cycle's input).  With x null the x update
+// is deferred to the p update of iteration k (which reads p anyway): the
+// update records k in st->x_iter.  Two entries per lane and step (16-byte
+// accesses); an odd last entry is handled by the first lane.
+__global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st, double* __restrict__ x,
                                                         const double* __restrict__ p, double* __restrict__ r,
                                                         const double* __restrict__ Ap,
                                                         const double* __restrict__ minv,
                                                         double* __restrict__ partials,
-                                                        float* __restrict__ r32) {
+                                                        float* __restrict__ r32, int k) {
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double a = st->alpha;
     double rr = 0.0, rz = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS) {
-        x[i] += a * p[i];
+    const int64_t n2 = n / 2;
+    const int64_t t0 = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x, ts = (int64_t)RED_BLOCKS * RED_THREADS;
+    for (int64_t i = t0; i < n2; i += ts) {
+        if (x) {
+            double2 xv = reinterpret_cast<double2*>(x)[i];
+            const double2 pv = reinterpret_cast<const double2*>(p)[i];
+            xv.x += a * pv.x;
+            xv.y += a * pv.y;
+            reinterpret_cast<double2*>(x)[i] = xv;
+        }
+        double2 rv = reinterpret_cast<double2*>(r)[i];
+        const double2 av = reinterpret_cast<const double2*>(Ap)[i];
+        rv.x -= a * av.x;
+        rv.y -= a * av.y;
+        reinterpret_cast<double2*>(r)[i] = rv;
+        if (r32) reinterpret_cast<float2*>(r32)[i] = make_float2((float)rv.x, (float)rv.y);
+        rr += rv.x * rv.x;
+        rr += rv.y * rv.y;
+        if (minv) {
+            const double2 mv = reinterpret_cast<const double2*>(minv)[i];
+            rz += rv.x * (rv.x * mv.x);
+            rz += rv.y * (rv.y * mv.y);
+        }
+    }
+    if ((n & 1) && t0 == 0) {
+        const int64_t i = n - 1;
+        if (x) x[i] += a * p[i];
         const double ri = r[i] - a * Ap[i];
         r[i] = ri;
         if (r32) r32[i] = (float)ri;
         rr += ri * ri;
         if (minv) rz += ri * (ri * minv[i]);
     }
+    if (!x && t0 == 0) st->x_iter = k;
     rr = block_sum(rr, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = rr;
     if (minv) {
@@ -248,12 +278,37 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_dots_z32(int64_t n, const 
     }
 }
 
-__global__ void k_p_update_z32(int64_t n, const PcgState* st, double* __restrict__ p, const float* __restrict__ z) {
-    if (st->halted) return;
-    const double b = st->beta;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = (double)z[i] + b * p[i];
+// p = z + beta p (z in FP32).  With x given, first the x update deferred
+// by k_update of iteration k: x += alpha p with the old p, also when this
+// iteration's check halted the loop.
+__global__ void k_p_update_z32(int64_t n, const PcgState* st, double* __restrict__ p, const float* __restrict__ z,
+                               double* __restrict__ x, int k) {
+    const bool upx = x && st->x_iter == k;
+    const bool upp = !st->halted;
+    if (!upx && !upp) return;
+    const double a = st->alpha, b = st->beta;
+    const int64_t n2 = n / 2;
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, ts = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = t0; i < n2; i += ts) {
+        double2 pv = reinterpret_cast<double2*>(p)[i];
+        if (upx) {
+            double2 xv = reinterpret_cast<double2*>(x)[i];
+            xv.x += a * pv.x;
+            xv.y += a * pv.y;
+            reinterpret_cast<double2*>(x)[i] = xv;
+        }
+        if (upp) {
+            const float2 zv = reinterpret_cast<const float2*>(z)[i];
+            pv.x = (double)zv.x + b * pv.x;
+            pv.y = (double)zv.y + b * pv.y;
+            reinterpret_cast<double2*>(p)[i] = pv;
+        }
+    }
+    if ((n & 1) && t0 == 0) {
+        const int64_t i = n - 1;
+        if (upx) x[i] += a * p[i];
+        if (upp) p[i] = (double)z[i] + b * p[i];
+    }
 }
 
 // p = z + beta p; with z == nullptr, z = minv * r (Jacobi, not stored).
@@ -298,8 +353,8 @@ void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int
     k_alpha_n<<<1, RED_THREADS, 0, s>>>(st, partials, n, breakdown_check);
 }
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32) {
-    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32);
+                       const double* Ap, const double* minv, double* partials, float* r32, int k) {
+    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32, k);
 }
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
                       int cpu_semantics, int jacobi) {
@@ -320,8 +375,9 @@ void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, cons
                               const double* Ap, double* partials) {
     k_flex_dots_z32<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials);
 }
-void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z) {
-    k_p_update_z32<<<grid_for(n), 256, 0, s>>>(n, st, p, z);
+void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z, double* x,
+                             int k) {
+    k_p_update_z32<<<grid_for((n + 1) / 2), 256, 0, s>>>(n, st, p, z, x, k);
 }
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best) {
     k_best_copy<<<grid_for(n), 256, 0, s>>>(n, st, x, best);
