@@ -10,6 +10,7 @@
 // in FP64 or in FP32 with BF16-stored smoother diagonals.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -135,6 +136,11 @@ bool mixed(const hexa_handle* h) { return h->opt.precision == HEXA_PREC_MIXED; }
 // Builds the hierarchy's shift-independent part: grids, Ke, factors, masks,
 // diagonals, work vectors.
 int32_t mg_build(hexa_handle* h) {
+    // the grid transfers index nodes with 32 bits
+    if (h->g.nn > (int64_t)UINT32_MAX) {
+        set_error("multigrid needs fewer than 2^32 nodes");
+        return HEXA_E_INVALID;
+    }
     const int L = choose_levels(h);
     const bool mx = mixed(h);
     h->levels.assign(L, Level{});
@@ -293,6 +299,10 @@ void apply_mg(hexa_handle* h, const double* r, double* z) {
 // inverse coarse diagonal from the x-entry diagonal of the coarse brick's
 // Float32 Ke with E = 1 and nu = 0.3 (as :299-302), an all-free coarse mask.
 int32_t refmg_build(hexa_handle* h) {
+    if (h->g.nn > (int64_t)UINT32_MAX) {
+        set_error("multigrid needs fewer than 2^32 nodes");
+        return HEXA_E_INVALID;
+    }
     if (h->refmg_ready) return HEXA_OK;
     const Grid3& f = h->g;
     h->rg = make_grid(std::max(1, f.nx / 2), std::max(1, f.ny / 2), std::max(1, f.nz / 2));

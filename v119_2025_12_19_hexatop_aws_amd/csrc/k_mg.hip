@@ -21,9 +21,20 @@ inline unsigned grid_for(int64_t n, int threads = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-// Transfers run node rows: threads along x, each block row walks a strided
-// set of (y, z) rows, so blocks live long enough to keep the memory busy.
-constexpr int TB = 128;
+// Transfers run one thread per output node over the flattened node index
+// (no idle lanes at row ends); the index is split into (i, j, k) with
+// multiply-high divisions.
+constexpr int TB = 256;
+
+// n / d for 0 <= n < 2^32 as the high word of n * ceil(2^64 / d)
+struct FastDiv {
+    uint64_t m;
+    uint32_t d;
+};
+inline FastDiv fast_div(uint32_t d) { return FastDiv{d > 1 ? ~0ull / d + 1 : 0, d}; }
+__device__ __forceinline__ uint32_t operator/(uint32_t n, const FastDiv& f) {
+    return f.d > 1 ? (uint32_t)__umul64hi((uint64_t)n, f.m) : n;
+}
 
 // A node's three components, loaded and stored as one vector access (12 or
 // 24 contiguous bytes per lane, lanes contiguous along x).
@@ -39,33 +50,31 @@ __device__ __forceinline__ void st(T* p, int64_t node, const V3<T>& v) { *reinte
 // r_c(I,J,K) = sum over the 3x3x3 fine nodes around (2I,2J,2K) of
 // w(di) w(dj) w(dk) r_f, w(0) = 1, w(+-1) = 1/2; held coarse DOFs -> 0.
 template <typename T>
-__global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, const T* __restrict__ rf,
-                                                 const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
-    const int CNX = c.nx + 1, CNY = c.ny + 1;
+__global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, FastDiv dnx, FastDiv dny, uint32_t ncn,
+                                                 const T* __restrict__ rf, const uint8_t* __restrict__ cmask,
+                                                 T* __restrict__ rc) {
+    const uint32_t n = blockIdx.x * TB + threadIdx.x;
+    if (n >= ncn) return;
+    const uint32_t row = n / dnx, K = row / dny;
+    const int I = (int)(n - row * dnx.d), J = (int)(row - K * dny.d);
     const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
-    const int I = blockIdx.x * TB + threadIdx.x;
-    if (I >= CNX) return;
-    const int rows = CNY * (c.nz + 1);
-    for (int rr = blockIdx.y; rr < rows; rr += gridDim.y) {
-    const int J = rr % CNY, K = rr / CNY;
-    const int64_t n = I + (int64_t)CNX * (J + (int64_t)CNY * K);
     const uint8_t mb = cmask[n];
     V3<T> s{T(0), T(0), T(0)};
     for (int dk = -1; dk <= 1; ++dk) {
-        const int k = 2 * K + dk;
+        const int k = 2 * (int)K + dk;
         if ((unsigned)k >= (unsigned)FNZ) continue;
         const T wk = dk == 0 ? T(1) : T(0.5);
         for (int dj = -1; dj <= 1; ++dj) {
             const int j = 2 * J + dj;
             if ((unsigned)j >= (unsigned)FNY) continue;
             const T wjk = wk * (dj == 0 ? T(1) : T(0.5));
-            const int64_t row = (int64_t)FNX * (j + (int64_t)FNY * k);
+            const int64_t frow = (int64_t)FNX * (j + (int64_t)FNY * k);
 #pragma unroll
             for (int di = -1; di <= 1; ++di) {
                 const int i = 2 * I + di;
                 if ((unsigned)i >= (unsigned)FNX) continue;
                 const T w = wjk * (di == 0 ? T(1) : T(0.5));
-                const V3<T> v = ld(rf, row + i);
+                const V3<T> v = ld(rf, frow + i);
                 s.x += w * v.x;
                 s.y += w * v.y;
                 s.z += w * v.z;
@@ -73,38 +82,35 @@ __global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, const T* __re
         }
     }
     st(rc, n, V3<T>{(mb & 1) ? T(0) : s.x, (mb & 2) ? T(0) : s.y, (mb & 4) ? T(0) : s.z});
-    }
 }
 
-// x_f += P x_c, masked by the fine supports; one thread per fine node, node
-// rows along the block.  Along each axis an even fine index coincides with
-// coarse node i/2 (weight 1), an odd one lies midway between (i-1)/2 and
-// (i+1)/2 (weights 1/2).  Coarse nodes beyond the coarse grid (the last fine
-// node of an odd axis under a floor(n/2) coarse grid) contribute nothing.
+// x_f += P x_c, masked by the fine supports; one thread per fine node.
+// Along each axis an even fine index coincides with coarse node i/2 (weight
+// 1), an odd one lies midway between (i-1)/2 and (i+1)/2 (weights 1/2).
+// Coarse nodes beyond the coarse grid (the last fine node of an odd axis
+// under a floor(n/2) coarse grid) contribute nothing.
 template <typename T>
-__global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, const T* __restrict__ xc,
-                                                    const uint8_t* __restrict__ fmask, T* __restrict__ xf) {
-    const int CNX = c.nx + 1, CNY = c.ny + 1, CNZ = c.nz + 1;
-    const int FNX = f.nx + 1, FNY = f.ny + 1;
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= FNX) return;
-    const int rows = FNY * (f.nz + 1);
-    for (int rr = blockIdx.y; rr < rows; rr += gridDim.y) {
-    const int j = rr % FNY, k = rr / FNY;
-    const int64_t n = i + (int64_t)FNX * (j + (int64_t)FNY * k);
+__global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, FastDiv dnx, FastDiv dny, uint32_t nfn,
+                                                    const T* __restrict__ xc, const uint8_t* __restrict__ fmask,
+                                                    T* __restrict__ xf) {
+    const uint32_t n = blockIdx.x * TB + threadIdx.x;
+    if (n >= nfn) return;
     const uint8_t mb = fmask[n];
-    if ((mb & 7) == 7) continue;
-    const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    if ((mb & 7) == 7) return;
+    const uint32_t row = n / dnx, k = row / dny;
+    const int i = (int)(n - row * dnx.d), j = (int)(row - k * dny.d);
+    const int CNX = c.nx + 1, CNY = c.ny + 1, CNZ = c.nz + 1;
+    const int i0 = i >> 1, j0 = j >> 1, k0 = (int)k >> 1;
     const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
     V3<T> s{T(0), T(0), T(0)};
     for (int a = 0; a < nk; ++a) {
         if (k0 + a >= CNZ) continue;
         for (int bb = 0; bb < nj; ++bb) {
             if (j0 + bb >= CNY) continue;
-            const int64_t row = (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
+            const int64_t crow = (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
             for (int cc = 0; cc < ni; ++cc) {
                 if (i0 + cc >= CNX) continue;
-                const V3<T> v = ld(xc, row + i0 + cc);
+                const V3<T> v = ld(xc, crow + i0 + cc);
                 s.x += v.x;
                 s.y += v.y;
                 s.z += v.z;
@@ -117,7 +123,6 @@ __global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, const T* _
     if (!(mb & 2)) x.y += w * s.y;
     if (!(mb & 4)) x.z += w * s.z;
     st(xf, n, x);
-    }
 }
 
 // Reference 2-level preconditioner, coarse diagonal
@@ -247,17 +252,17 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
 template <typename T>
 void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
                      const uint8_t* cmask, T* rc) {
-    const int64_t rows = (int64_t)(coarse.ny + 1) * (coarse.nz + 1);
-    const dim3 grid((unsigned)((coarse.nx + 1 + TB - 1) / TB), (unsigned)std::min<int64_t>(rows, 4096));
-    k_restrict<T><<<grid, TB, 0, s>>>(fine, coarse, rf, cmask, rc);
+    const uint32_t ncn = (uint32_t)coarse.nn;
+    k_restrict<T><<<(ncn + TB - 1) / TB, TB, 0, s>>>(fine, coarse, fast_div(coarse.nx + 1), fast_div(coarse.ny + 1),
+                                                     ncn, rf, cmask, rc);
 }
 
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf) {
-    const int64_t rows = (int64_t)(fine.ny + 1) * (fine.nz + 1);
-    const dim3 grid((unsigned)((fine.nx + 1 + TB - 1) / TB), (unsigned)std::min<int64_t>(rows, 4096));
-    k_prolong_add<T><<<grid, TB, 0, s>>>(fine, coarse, xc, fmask, xf);
+    const uint32_t nfn = (uint32_t)fine.nn;
+    k_prolong_add<T><<<(nfn + TB - 1) / TB, TB, 0, s>>>(fine, coarse, fast_div(fine.nx + 1), fast_div(fine.ny + 1),
+                                                        nfn, xc, fmask, xf);
 }
 
 void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
