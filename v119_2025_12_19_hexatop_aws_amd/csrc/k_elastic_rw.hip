@@ -454,9 +454,9 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
 // covers nodes 62 tx .. 62 tx + 63: tile 0 owns lanes 0..62, the others lanes
 // 1..62 (lane 0 is the halo node shared with the tile to the left), so a row
 // segment never starts before its row and every 16-byte chunk of it is read
-// from its own plane.  Nodes past the row end read the next row's values (or
-// zeros past the plane end); they only meet elements whose factor is forced
-// to zero.
+// from its own plane.  Lanes past the row end hold the next row's values (or
+// zeros past the plane end); elements there have their factor forced to
+// zero, and the row's last node takes its +x neighbour as zero.
 //
 // Ordering: the loads to LDS are counted by vmcnt like any other vector
 // memory access, in issue order.  Every layer issues at least MIN_OPS
@@ -581,11 +581,20 @@ __global__ __launch_bounds__(WL* DmaGeom<T>::BY) void k_elastic_dma(
     struct Quad {
         T a[3], bx[3], cy[3], d[3];
     };
+    // (the +x neighbours of the last node of a row are values of the next
+    // row; they meet only a zero element factor, but are zeroed so that an
+    // Inf or NaN there cannot reach the row's last node)
+    const bool xin = i < g.nx;
     auto fetch = [&](int k, Quad& n) {
         node(k, w, l, n.a);
         node(k, w, lx, n.bx);
         node(k, w + 1, l, n.cy);
         node(k, w + 1, lx, n.d);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            n.bx[c] = xin ? n.bx[c] : T(0);
+            n.d[c] = xin ? n.d[c] : T(0);
+        }
     };
     // x-y modes of a fetched plane, and this lane's own node values
     auto plane_modes = [&](const Quad& n, T (&P)[3][4], T (&own)[3]) {
