@@ -154,7 +154,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
-    ap.add_argument("--jacobi-max-iter", type=int, default=30000)
+    ap.add_argument("--jacobi-max-iter", type=int, default=3000)
     ap.add_argument("--serve-jobs", type=int, default=40, help="jobs of the serving stream (0 = skip)")
     ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--no-load-cases", action="store_true", help="skip the 8-load-case solve")
