@@ -1,0 +1,62 @@
+"""GPU tests of the graph-replayed PCG iteration: solves replayed from the
+captured iteration graph equal the eagerly launched solves bit for bit (same
+iterations, same u), for every preconditioner and both semantics, across
+repeated solves on one handle with eager work between them."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from v119_2025_12_19_hexatop_aws_amd import hexa
+from v119_2025_12_19_hexatop_aws_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _solve(s, precond, semantics, tol):
+    u, reps, st = s.solve(tol=tol, precond=precond, semantics=semantics, max_iter=3000)
+    if st == hexa.OK:
+        # the returned u is consistent with the residual the loop stopped on
+        # (iterations queued behind the halt change nothing)
+        assert reps[0]["true_rel_res"] < 1.01 * tol, reps[0]
+    return u[0].copy(), reps[0]["iterations"], st
+
+
+@pytest.mark.parametrize("precision,precond,semantics", [
+    (hexa.PREC_MIXED, hexa.PRECOND_MG, hexa.SEMANTICS_CPU),
+    (hexa.PREC_FP64, hexa.PRECOND_MG, hexa.SEMANTICS_GPU),
+    (hexa.PREC_FP64, hexa.PRECOND_JACOBI, hexa.SEMANTICS_CPU),
+    (hexa.PREC_FP64, hexa.PRECOND_REF_MG, hexa.SEMANTICS_GPU)])
+def test_graph_replay_equals_eager(precision, precond, semantics):
+    say("graph", precision, precond, semantics)
+    p = P.cantilever(40, 16, 16, seed=9, smooth_passes=2)
+    tol = 1e-8
+    old = os.environ.get("HEXA_GRAPHS")
+    try:
+        with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
+            s.set_problem(p.fixed, p.f[0])
+            s.set_density(p.rho)
+            os.environ["HEXA_GRAPHS"] = "0"
+            ue, ke, ste = _solve(s, precond, semantics, tol)
+            os.environ.pop("HEXA_GRAPHS")
+            ug1, kg1, stg1 = _solve(s, precond, semantics, tol)
+            y = s.matvec(ug1)                      # eager work between the replays
+            ug2, kg2, stg2 = _solve(s, precond, semantics, tol)
+            os.environ["HEXA_GRAPHS"] = "0"
+            ue2, ke2, ste2 = _solve(s, precond, semantics, tol)
+    finally:
+        if old is None:
+            os.environ.pop("HEXA_GRAPHS", None)
+        else:
+            os.environ["HEXA_GRAPHS"] = old
+    say("iterations", ke, kg1, kg2, ke2, "status", ste, stg1, stg2)
+    assert ste == stg1 == stg2 == ste2
+    assert ke == kg1 == kg2 == ke2 and ke > 1
+    assert np.array_equal(ue, ug1) and np.array_equal(ue, ug2) and np.array_equal(ue, ue2)
+    assert np.all(np.isfinite(y))

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -136,6 +137,7 @@ bool mixed(const hexa_handle* h) { return h->opt.precision == HEXA_PREC_MIXED; }
 // Builds the hierarchy's shift-independent part: grids, Ke, factors, masks,
 // diagonals, work vectors.
 int32_t mg_build(hexa_handle* h) {
+    ++h->graph_gen;
     // the grid transfers index nodes with 32 bits
     if (h->g.nn > (int64_t)UINT32_MAX) {
         set_error("multigrid needs fewer than 2^32 nodes");
@@ -299,6 +301,7 @@ void apply_mg(hexa_handle* h, const double* r, double* z) {
 // inverse coarse diagonal from the x-entry diagonal of the coarse brick's
 // Float32 Ke with E = 1 and nu = 0.3 (as :299-302), an all-free coarse mask.
 int32_t refmg_build(hexa_handle* h) {
+    ++h->graph_gen;
     if (h->g.nn > (int64_t)UINT32_MAX) {
         set_error("multigrid needs fewer than 2^32 nodes");
         return HEXA_E_INVALID;
@@ -390,11 +393,13 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
     // there, with CPU semantics (no best-iterate copy of x after the check),
     // x += alpha p moves from the update to the p update, which reads p anyway
     const bool defer_x = mx && cpu;
-    for (int k = 1; k <= prm->max_iter && !stop; ++k) {
+    // One iteration.  Everything that changes between iterations (the
+    // iteration count, the check schedule, the deferred x update) is decided
+    // on the device, so the launches are the same every iteration.
+    auto iteration = [&]() -> int32_t {
         // Ap = (K + sI) p, with the per-block partials of (p, Ap)
         const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask,
                                                h->d_p, nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart);
-        ++*matvecs;
         if (nq > 0) {
             launch_pcg_alpha_n(st, h->d_state, h->d_dotpart, nq, cpu ? 0 : 1);
         } else {
@@ -402,20 +407,21 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
             launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
         }
         launch_pcg_update(st, nd, h->d_state, defer_x ? nullptr : h->d_x, h->d_p, h->d_r, h->d_Ap,
-                          pk == 0 ? h->d_minv : nullptr, h->d_partials, r32, k);
-        const int check = cpu ? 1 : ((k % check_every == 0) || k == 1);
+                          pk == 0 ? h->d_minv : nullptr, h->d_partials, r32);
         if (pk == 2) {
             // z = M r and (r, z) for the standard beta (GPUSolver.jl:409-417)
             apply_refmg(h, h->d_r, h->d_z);
             launch_pcg_rz(st, nd, h->d_r, h->d_z, h->d_partials + RED_BLOCKS);
         }
-        launch_pcg_check(st, h->d_state, h->d_partials, k, check, cpu ? 1 : 0, use_mg ? 0 : 1);
-        if (check && !cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
+        launch_pcg_check(st, h->d_state, h->d_partials, cpu ? 1 : check_every, cpu ? 0 : 1, cpu ? 1 : 0,
+                         use_mg ? 0 : 1);
+        // (copies only when the check improved the best residual)
+        if (!cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
         if (mx) {
             vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32);
             launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_partials);
             launch_pcg_flex_beta(st, h->d_state, h->d_partials);
-            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, defer_x ? h->d_x : nullptr, k);
+            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, defer_x ? h->d_x : nullptr);
         } else if (use_mg) {
             apply_mg(h, h->d_r, h->d_z);
             launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);
@@ -426,7 +432,46 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
         } else {
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, nullptr, h->d_r, h->d_minv);
         }
-        HX_TRY(check_launch());
+        return check_launch();
+    };
+    // Replayed as a graph (one launch per iteration instead of tens; what
+    // small grids are bound by), captured once per configuration.
+    // HEXA_GRAPHS=0 launches eagerly.
+    const char* ge = getenv("HEXA_GRAPHS");
+    const bool graphs = !(ge && ge[0] == '0');
+    if (graphs && !(h->pcg_exec && h->pcg_key.pk == pk && h->pcg_key.cpu == (int)cpu &&
+                    h->pcg_key.every == check_every && h->pcg_key.s == s && h->pcg_key.gen == h->graph_gen)) {
+        if (h->pcg_exec) {
+            HX_CHECK(hipGraphExecDestroy(h->pcg_exec));
+            h->pcg_exec = nullptr;
+        }
+        HX_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int32_t rc = iteration();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc != HEXA_OK || e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (rc != HEXA_OK) return rc;
+            set_error("PCG iteration capture failed: %s", hipGetErrorString(e));
+            return HEXA_E_DEVICE;
+        }
+        const hipError_t ei = hipGraphInstantiate(&h->pcg_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            h->pcg_exec = nullptr;
+            set_error("PCG iteration graph instantiation failed: %s", hipGetErrorString(ei));
+            return HEXA_E_DEVICE;
+        }
+        h->pcg_key.pk = pk;
+        h->pcg_key.cpu = (int)cpu;
+        h->pcg_key.every = check_every;
+        h->pcg_key.s = s;
+        h->pcg_key.gen = h->graph_gen;
+    }
+    for (int k = 1; k <= prm->max_iter && !stop; ++k) {
+        if (graphs) HX_CHECK(hipGraphLaunch(h->pcg_exec, st));
+        else HX_TRY(iteration());
+        ++*matvecs;
         k_done = k;
         HX_TRY(poll_state(h, k, &stop));
     }
@@ -762,6 +807,7 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
 int32_t hexa_destroy(hexa_handle* h) {
     if (!h) return HEXA_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
     for (void* p : h->allocs) hipFree(p);
     if (h->h_state) hipHostFree(h->h_state);
     for (int i = 0; i < RING; ++i)

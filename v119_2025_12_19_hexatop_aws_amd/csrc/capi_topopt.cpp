@@ -63,7 +63,7 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
         launch_pcg_pAp(st, nn, h->d_state, h->d_hp, h->d_hAp, h->d_partials);
         launch_pcg_alpha(st, h->d_state, h->d_partials, 0);
         launch_pcg_update(st, nn, h->d_state, h->d_hx, h->d_hp, h->d_hr, h->d_hAp, h->d_hdinv, h->d_partials);
-        launch_pcg_check(st, h->d_state, h->d_partials, k, check_every > 0 && k % check_every == 0, 1, 1);
+        launch_pcg_check(st, h->d_state, h->d_partials, check_every, 0, 1, 1);
         launch_pcg_p_update(st, nn, h->d_state, h->d_hp, nullptr, h->d_hr, h->d_hdinv);
         HX_TRY(check_launch());
         k_done = k;

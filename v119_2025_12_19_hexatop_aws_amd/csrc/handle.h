@@ -45,6 +45,15 @@ struct hexa_handle {
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     hipEvent_t ring_ev[hexa::RING] = {};
     double last_kernel_ms = 0.0;
+    // one PCG iteration captured as a graph, for the configuration in
+    // pcg_key; graph_gen advances whenever a preconditioner is rebuilt
+    hipGraphExec_t pcg_exec = nullptr;
+    struct {
+        int pk = -1, cpu = 0, every = 0;
+        double s = 0.0;
+        int64_t gen = -1;
+    } pcg_key;
+    int64_t graph_gen = 0;
 
     double Ke[576];
     bool factored = false;          // exact brick Ke -> factored element kernel

@@ -93,6 +93,8 @@ struct PcgState {
     int32_t improved;  // last check improved best_rel
     int32_t last_k;    // last iteration that updated x
     int32_t x_iter;    // iteration whose x update k_update deferred
+    int32_t k;         // iterations started (advanced by the alpha kernels)
+    int32_t x_done;    // last iteration whose deferred x update was applied
 };
 
 // ---- kernel launchers (implemented in the .hip files) -----------------
@@ -209,8 +211,10 @@ void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, con
                     double* partials);
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check);
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32 = nullptr, int k = 0);
-void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
+                       const double* Ap, const double* minv, double* partials, float* r32 = nullptr);
+// The residual test runs when k % check_every == 0 (check_every > 0), or
+// k == 1 with check_first; k is the state's iteration count.
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int check_every, int check_first,
                       int cpu_semantics, int jacobi);
 void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
                           const double* Ap, double* partials);
@@ -221,7 +225,7 @@ void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int
 void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
                               const double* Ap, double* partials);
 void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z,
-                             double* x = nullptr, int k = 0);
+                             double* x = nullptr);
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best);
 
 }  // namespace hexa

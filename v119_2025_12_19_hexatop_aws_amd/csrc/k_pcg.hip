@@ -58,6 +58,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_init_state(PcgState* st, const 
         st->improved = 0;
         st->last_k = 0;
         st->x_iter = 0;
+        st->x_done = 0;
+        st->k = 0;
     }
 }
 
@@ -95,10 +97,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_pAp(int64_t n, const PcgState* 
 // (GPUSolver.jl:373-375).
 __global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const double* partials,
                                                        int breakdown_check) {
+    // the previous iteration's deferred x update (if any) has been applied
+    if (threadIdx.x == 0) st->x_done = st->x_iter;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double den = final_sum(partials, 0, lds);
     if (threadIdx.x == 0) {
+        st->k += 1;   // one alpha per iteration: the device's iteration count
         st->pAp = den;
         if (breakdown_check && fabs(den) < 1e-20) {
             st->halted = PCG_BREAKDOWN;
@@ -112,10 +117,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const doubl
 // alpha from n per-block partials of (p, Ap) written by the operator kernel.
 __global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const double* partials, int n,
                                                          int breakdown_check) {
+    if (threadIdx.x == 0) st->x_done = st->x_iter;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double den = final_sum_n(partials, n, lds);
     if (threadIdx.x == 0) {
+        st->k += 1;
         st->pAp = den;
         if (breakdown_check && fabs(den) < 1e-20) {
             st->halted = PCG_BREAKDOWN;
@@ -129,15 +136,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const dou
 // x += alpha p; r -= alpha Ap; partials: (r,r) and, when minv is given,
 // (r, minv r) for the Jacobi preconditioner.  r32, when given, receives the
 // new residual in FP32 (the mixed V-cycle's input).  With x null the x update
-// is deferred to the p update of iteration k (which reads p anyway): the
-// update records k in st->x_iter.  Two entries per lane and step (16-byte
+// is deferred to the p update of the same iteration (which reads p anyway):
+// the update records the iteration st->k in st->x_iter.  Two entries per lane and step (16-byte
 // accesses); an odd last entry is handled by the first lane.
 __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st, double* __restrict__ x,
                                                         const double* __restrict__ p, double* __restrict__ r,
                                                         const double* __restrict__ Ap,
                                                         const double* __restrict__ minv,
                                                         double* __restrict__ partials,
-                                                        float* __restrict__ r32, int k) {
+                                                        float* __restrict__ r32) {
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double a = st->alpha;
@@ -175,7 +182,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st,
         rr += ri * ri;
         if (minv) rz += ri * (ri * minv[i]);
     }
-    if (!x && t0 == 0) st->x_iter = k;
+    if (!x && t0 == 0) st->x_iter = st->k;
     rr = block_sum(rr, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = rr;
     if (minv) {
@@ -190,9 +197,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st,
 // rel > 10 or (k > 500 and rel > 100 best) -> diverged.  CPU semantics
 // (CPUSolver.jl:233-239): rel < tol every iteration, nothing else.
 // For the Jacobi preconditioner beta = (r,z)_new/(r,z) is formed here too.
-__global__ __launch_bounds__(RED_THREADS) void k_check(PcgState* st, const double* partials, int k,
-                                                       int check, int cpu_semantics, int jacobi) {
+__global__ __launch_bounds__(RED_THREADS) void k_check(PcgState* st, const double* partials, int check_every,
+                                                       int check_first, int cpu_semantics, int jacobi) {
     if (st->halted) return;
+    const int k = st->k;
+    const bool check = (check_every > 0 && k % check_every == 0) || (check_first && k == 1);
     __shared__ double lds[RED_THREADS / 64];
     const double rr = final_sum(partials, 0, lds);
     const double rzn = jacobi ? final_sum(partials, 1, lds) : 0.0;
@@ -279,11 +288,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_dots_z32(int64_t n, const 
 }
 
 // p = z + beta p (z in FP32).  With x given, first the x update deferred
-// by k_update of iteration k: x += alpha p with the old p, also when this
-// iteration's check halted the loop.
+// by k_update of this iteration: x += alpha p with the old p, also when this
+// iteration's check halted the loop, and once only (the iterations launched
+// after a halt neither advance k nor repeat it: the alpha kernel records it
+// as done).
 __global__ void k_p_update_z32(int64_t n, const PcgState* st, double* __restrict__ p, const float* __restrict__ z,
-                               double* __restrict__ x, int k) {
-    const bool upx = x && st->x_iter == k;
+                               double* __restrict__ x) {
+    const bool upx = x && st->x_iter == st->k && st->x_done != st->k;
     const bool upp = !st->halted;
     if (!upx && !upp) return;
     const double a = st->alpha, b = st->beta;
@@ -353,12 +364,12 @@ void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int
     k_alpha_n<<<1, RED_THREADS, 0, s>>>(st, partials, n, breakdown_check);
 }
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32, int k) {
-    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32, k);
+                       const double* Ap, const double* minv, double* partials, float* r32) {
+    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32);
 }
-void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int k, int check,
+void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int check_every, int check_first,
                       int cpu_semantics, int jacobi) {
-    k_check<<<1, RED_THREADS, 0, s>>>(st, partials, k, check, cpu_semantics, jacobi);
+    k_check<<<1, RED_THREADS, 0, s>>>(st, partials, check_every, check_first, cpu_semantics, jacobi);
 }
 void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
                           const double* Ap, double* partials) {
@@ -375,9 +386,8 @@ void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, cons
                               const double* Ap, double* partials) {
     k_flex_dots_z32<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials);
 }
-void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z, double* x,
-                             int k) {
-    k_p_update_z32<<<grid_for((n + 1) / 2), 256, 0, s>>>(n, st, p, z, x, k);
+void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z, double* x) {
+    k_p_update_z32<<<grid_for((n + 1) / 2), 256, 0, s>>>(n, st, p, z, x);
 }
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best) {
     k_best_copy<<<grid_for(n), 256, 0, s>>>(n, st, x, best);
