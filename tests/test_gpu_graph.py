@@ -60,3 +60,28 @@ def test_graph_replay_equals_eager(precision, precond, semantics):
     assert ke == kg1 == kg2 == ke2 and ke > 1
     assert np.array_equal(ue, ug1) and np.array_equal(ue, ug2) and np.array_equal(ue, ue2)
     assert np.all(np.isfinite(y))
+
+
+def test_filter_graph_replay_equals_eager():
+    """The Helmholtz-filter PCG replayed from its captured iteration equals
+    the eager loop bit for bit (iterations, residual, field)."""
+    say("filter graph")
+    p = P.cantilever(40, 16, 16, seed=10)
+    old = os.environ.get("HEXA_GRAPHS")
+    out = []
+    try:
+        with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0) as s:
+            for flag in ("0", None, None, "0"):
+                if flag is None:
+                    os.environ.pop("HEXA_GRAPHS", None)
+                else:
+                    os.environ["HEXA_GRAPHS"] = flag
+                out.append(s.filter(p.rho, 1.5, tol=1e-10, max_iter=2000))
+    finally:
+        if old is None:
+            os.environ.pop("HEXA_GRAPHS", None)
+        else:
+            os.environ["HEXA_GRAPHS"] = old
+    say("iterations", [o[1] for o in out])
+    assert all(o[1] == out[0][1] and o[2] == out[0][2] for o in out) and out[0][1] > 1
+    assert all(np.array_equal(o[0], out[0][0]) for o in out)

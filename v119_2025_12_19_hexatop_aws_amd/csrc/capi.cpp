@@ -808,6 +808,7 @@ int32_t hexa_destroy(hexa_handle* h) {
     if (!h) return HEXA_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
+    if (h->filt_exec) hipGraphExecDestroy(h->filt_exec);
     for (void* p : h->allocs) hipFree(p);
     if (h->h_state) hipHostFree(h->h_state);
     for (int i = 0; i < RING; ++i)

@@ -2,6 +2,7 @@
 // sensitivity, optimality-criteria update, one full SIMP/OC iteration, and
 // the reference's stress-ratio density update.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -58,14 +59,46 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     launch_pcg_set_rz(st, h->d_state, h->d_partials);
     int k_done = 0;
     bool stop = false;
-    for (int k = 1; k <= max_iter && !stop; ++k) {
+    auto iteration = [&]() -> int32_t {
         launch_helm_apply(st, g, h->d_Ae, h->d_hp, h->d_hAp);
         launch_pcg_pAp(st, nn, h->d_state, h->d_hp, h->d_hAp, h->d_partials);
         launch_pcg_alpha(st, h->d_state, h->d_partials, 0);
         launch_pcg_update(st, nn, h->d_state, h->d_hx, h->d_hp, h->d_hr, h->d_hAp, h->d_hdinv, h->d_partials);
         launch_pcg_check(st, h->d_state, h->d_partials, check_every, 0, 1, 1);
         launch_pcg_p_update(st, nn, h->d_state, h->d_hp, nullptr, h->d_hr, h->d_hdinv);
-        HX_TRY(check_launch());
+        return check_launch();
+    };
+    // replayed from a captured graph, as the elasticity PCG (the buffers are
+    // the handle's own, so only the check period keys it)
+    const char* ge = getenv("HEXA_GRAPHS");
+    const bool graphs = !(ge && ge[0] == '0');
+    if (graphs && !(h->filt_exec && h->filt_every == check_every)) {
+        if (h->filt_exec) {
+            HX_CHECK(hipGraphExecDestroy(h->filt_exec));
+            h->filt_exec = nullptr;
+        }
+        HX_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int32_t rc = iteration();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc != HEXA_OK || e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (rc != HEXA_OK) return rc;
+            set_error("filter iteration capture failed: %s", hipGetErrorString(e));
+            return HEXA_E_DEVICE;
+        }
+        const hipError_t ei = hipGraphInstantiate(&h->filt_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            h->filt_exec = nullptr;
+            set_error("filter iteration graph instantiation failed: %s", hipGetErrorString(ei));
+            return HEXA_E_DEVICE;
+        }
+        h->filt_every = check_every;
+    }
+    for (int k = 1; k <= max_iter && !stop; ++k) {
+        if (graphs) HX_CHECK(hipGraphLaunch(h->filt_exec, st));
+        else HX_TRY(iteration());
         k_done = k;
         HX_TRY(poll_state(h, k, &stop));
     }

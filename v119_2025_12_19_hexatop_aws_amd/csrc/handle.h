@@ -54,6 +54,8 @@ struct hexa_handle {
         int64_t gen = -1;
     } pcg_key;
     int64_t graph_gen = 0;
+    hipGraphExec_t filt_exec = nullptr;   // one Helmholtz-filter PCG iteration
+    int filt_every = 0;
 
     double Ke[576];
     bool factored = false;          // exact brick Ke -> factored element kernel

@@ -43,6 +43,9 @@ namespace {
 #ifndef HEXA_USE_DMA
 #define HEXA_USE_DMA 1
 #endif
+#ifndef HEXA_DMA_BY
+#define HEXA_DMA_BY 8
+#endif
 constexpr int WL = 64;           // lanes per wavefront = node columns per tile
 constexpr int OWN_X = WL - 2;    // owned node columns per tile
 constexpr int BY = HEXA_RW_BY;   // wavefronts per block = node rows per tile
@@ -472,7 +475,7 @@ template <typename T>
 struct DmaGeom {
     static constexpr int D = sizeof(T) == 8 ? 5 : 6;           // planes staged ahead of the layer
     static constexpr int NB = D + 1;                            // plane buffers
-    static constexpr int BY = 8;                               // waves per block
+    static constexpr int BY = HEXA_DMA_BY;                     // waves per block
     static constexpr int RCH = WL * 3 * (int)sizeof(T) / 16;  // 16-byte chunks per node row
     static constexpr int NC = (BY + 1) * RCH;                   // chunks per staged plane
     static constexpr int ROUNDS = (NC + WL * BY - 1) / (WL * BY);
