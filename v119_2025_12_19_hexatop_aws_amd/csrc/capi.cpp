@@ -537,6 +537,20 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     }
     // the first attempt's smoother and coarse inverse belong to the set-up
     if (use_mg && mg_kind == 1) HX_TRY(mg_set_shift(h, prm->shift_factor * maxd));
+    // z-chunk lengths of the operator kernels, timed once per grid and kernel
+    // (d_x, d_best and the level buffers are scratch at this point)
+    if (h->factored) {
+        tune_elastic_rw<double>(st, h->g, h->coef64, h->d_fac, h->d_x, h->d_best);
+        if (use_mg && mg_kind == 1)
+            for (size_t l = 0; l + 1 < h->levels.size(); ++l) {
+                Level& lv = h->levels[l];
+                if (mixed(h))
+                    tune_elastic_rw<float>(st, lv.g, lv.c32, lv.fac32, (const float*)lv.r, (float*)lv.xt);
+                else
+                    tune_elastic_rw<double>(st, lv.g, lv.c64, lv.fac64, (const double*)lv.r, (double*)lv.xt);
+            }
+        HX_TRY(check_launch());
+    }
     // warm start: from the host buffer, or (u_inout == NULL) from the
     // device-resident solution of the previous solve on this handle
     const bool warm = prm->warm_start && (u_inout || h->have_solution);
@@ -948,6 +962,14 @@ int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t prec
     if (!h->have_density) { set_error("hexa_set_density has not been called"); return HEXA_E_INVALID; }
     hipStream_t st = h->stream;
     if (precision == HEXA_PREC_MIXED) HX_TRY(ensure_fac32(h));
+    // (the chunk tuning reads x and writes y, as the product does)
+    if (h->factored) {
+        if (precision == HEXA_PREC_MIXED)
+            tune_elastic_rw<float>(st, h->g, h->coef32, h->d_fac32, (const float*)d_x, (float*)d_y);
+        else
+            tune_elastic_rw<double>(st, h->g, h->coef64, h->d_fac, (const double*)d_x, (double*)d_y);
+        HX_TRY(check_launch());
+    }
     HX_CHECK(hipEventRecord(h->ev_a, st));
     for (int r = 0; r < reps; ++r) {
         if (precision == HEXA_PREC_MIXED)

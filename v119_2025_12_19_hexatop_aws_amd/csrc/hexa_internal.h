@@ -118,6 +118,10 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
 bool elastic_rw_supported(const Grid3& g, size_t elem_bytes);
 // Upper bound on the partial sums launch_elastic_rw writes for grid g.
 int64_t elastic_rw_blocks(const Grid3& g);
+// Picks the z-chunk length of the row-wave kernels for grid g by timing
+// (process-wide, once per grid and kernel); u is read, y overwritten.
+template <typename T>
+void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, const T* fac, const T* u, T* y);
 template <typename T>
 FactCoef<T> make_fact_coef(double hx, double hy, double hz, double nu);
 template <typename T>

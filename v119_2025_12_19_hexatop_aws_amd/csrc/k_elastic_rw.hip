@@ -29,7 +29,11 @@
 // Every node sums its contributions in a fixed order: no atomics and the same
 // result on every run.
 #include <algorithm>
+#include <array>
 #include <climits>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "hexa_internal.h"
@@ -771,15 +775,31 @@ namespace {
 struct RwShape {
     int tiles_x, tiles_y, tiles_z, kchunk;
 };
-RwShape rw_shape(const Grid3& g) {
+
+// Chunk lengths (node planes per block along z) measured fastest for a grid
+// and kernel (tune_elastic_rw), shared by all handles of the process; grids
+// not tuned use the block-count rule below.
+enum TuneKind { TK_RW64 = 0, TK_RW32 = 1, TK_DMA32 = 2 };
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, int> g_tuned;
+int tuned_chunk(const Grid3& g, int kind) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tuned.find({g.nx, g.ny, g.nz, kind});
+    return it == g_tuned.end() ? 0 : it->second;
+}
+
+RwShape rw_shape(const Grid3& g, int kind = TK_RW64, int kchunk = 0) {
     RwShape r;
     r.tiles_x = (g.nx + 1 + OWN_X - 1) / OWN_X;
     r.tiles_y = (g.ny + 1 + OWN_Y - 1) / OWN_Y;
     // node planes per block: enough blocks to fill the chip, long enough
     // chunks to amortise the one recomputed element layer per block
-    r.kchunk = 32;
-    while (r.kchunk > 4 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
-        r.kchunk /= 2;
+    r.kchunk = kchunk ? kchunk : tuned_chunk(g, kind);
+    if (!r.kchunk) {
+        r.kchunk = 32;
+        while (r.kchunk > 4 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
+            r.kchunk /= 2;
+    }
     r.tiles_z = (g.nz + 1 + r.kchunk - 1) / r.kchunk;
     return r;
 }
@@ -791,19 +811,24 @@ bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
 
 // tiling of k_elastic_dma: tile tx starts at node 62 tx; odd chunks of
 // node planes give an even number of element layers per block
-RwShape dma_shape(const Grid3& g, int by) {
+RwShape dma_shape(const Grid3& g, int by, int kchunk = 0) {
     RwShape r;
     r.tiles_x = std::max(1, (g.nx + OWN_X - 1) / OWN_X);
     r.tiles_y = (g.ny + 1 + by - 2) / (by - 1);
-    r.kchunk = 31;
-    while (r.kchunk > 3 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
-        r.kchunk /= 2;
+    r.kchunk = kchunk ? kchunk : tuned_chunk(g, TK_DMA32);
+    if (!r.kchunk) {
+        r.kchunk = 31;
+        while (r.kchunk > 3 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
+            r.kchunk /= 2;
+    }
     r.tiles_z = (g.nz + 1 + r.kchunk - 1) / r.kchunk;
     return r;
 }
 
+// (the shortest chunks any tiling uses: the most blocks)
 int64_t elastic_rw_blocks(const Grid3& g) {
-    const RwShape r = rw_shape(g), a = dma_shape(g, DmaGeom<double>::BY), f = dma_shape(g, DmaGeom<float>::BY);
+    const RwShape r = rw_shape(g, TK_RW64, 4), a = dma_shape(g, DmaGeom<double>::BY, 3),
+                  f = dma_shape(g, DmaGeom<float>::BY, 3);
     return std::max({(int64_t)r.tiles_x * r.tiles_y * r.tiles_z, (int64_t)a.tiles_x * a.tiles_y * a.tiles_z,
                      (int64_t)f.tiles_x * f.tiles_y * f.tiles_z});
 }
@@ -837,7 +862,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
         return dot_part ? tiles : 0;
     }
 #endif
-    const RwShape r = rw_shape(g);
+    const RwShape r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32);
     const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
     dim3 block(WL, BY);
     switch (mode) {
@@ -860,6 +885,74 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
     }
     return dot_part ? tiles : 0;
 }
+
+// Times K(rho)u (APPLY) for each candidate chunk length on this grid and
+// records the fastest for the kernels of precision T (FP64: the register
+// kernel; FP32: the staged kernel and the register kernel of the first
+// smoothing sweep).  u is read and y overwritten.  Grids already tuned, too
+// small to matter, or HEXA_TUNE=0 keep the block-count rule.  The chunk
+// length changes the order of the per-block (u, Au) partial sums, so results
+// can differ in the last bits between a tuned and an untuned run.
+template <typename T>
+void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, const T* fac, const T* u, T* y) {
+    const char* te = getenv("HEXA_TUNE");
+    if ((te && te[0] == '0') || (int64_t)g.nx * g.ny * g.nz < 32768 || !elastic_rw_supported(g, sizeof(T))) return;
+    const int kinds[2] = {sizeof(T) == 8 ? (int)TK_RW64 : (int)TK_DMA32, (int)TK_RW32};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return;
+    }
+    for (int kk = 0; kk < (sizeof(T) == 8 ? 1 : 2); ++kk) {
+        const int kind = kinds[kk];
+        if (tuned_chunk(g, kind)) continue;
+        const bool dma = kind == TK_DMA32;
+        int best = 0;
+        float best_ms = 0.0f;
+        for (const int kc : dma ? std::array<int, 4>{31, 15, 7, 3} : std::array<int, 4>{32, 16, 8, 4}) {
+            const RwShape r = dma ? dma_shape(g, DmaGeom<float>::BY, kc) : rw_shape(g, kind, kc);
+            const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
+            auto launch = [&]() {
+                if constexpr (sizeof(T) == 4) {
+                    if (dma) {
+                        k_elastic_dma<T, T, OP_APPLY><<<tiles, dim3(WL, DmaGeom<T>::BY), 0, s>>>(
+                            g, coef, fac, nullptr, u, nullptr, nullptr, T(0), T(0), y, r.kchunk, r.tiles_x,
+                            r.tiles_y, tiles, nullptr);
+                        return;
+                    }
+                }
+                k_elastic_rw<T, T, OP_APPLY><<<tiles, dim3(WL, BY), 0, s>>>(g, coef, fac, nullptr, u, nullptr,
+                                                                            nullptr, T(0), T(0), y, r.kchunk,
+                                                                            r.tiles_x, r.tiles_y, tiles, nullptr);
+            };
+            launch();
+            launch();
+            hipEventRecord(e0, s);
+            for (int rep = 0; rep < 3; ++rep) launch();
+            hipEventRecord(e1, s);
+            float ms = 0.0f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                best = 0;
+                break;
+            }
+            if (!best || ms < best_ms) {
+                best = kc;
+                best_ms = ms;
+            }
+        }
+        if (best) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[{g.nx, g.ny, g.nz, kind}] = best;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+}
+template void tune_elastic_rw<double>(hipStream_t, const Grid3&, const FactCoef<double>&, const double*,
+                                      const double*, double*);
+template void tune_elastic_rw<float>(hipStream_t, const Grid3&, const FactCoef<float>&, const float*, const float*,
+                                     float*);
 
 #define HX_INST_RW(T, DT)                                                                            \
     template int launch_elastic_rw<T, DT>(hipStream_t, const Grid3&, OpMode, const FactCoef<T>&,     \
