@@ -44,12 +44,6 @@ namespace {
 #ifndef HEXA_RW_BY
 #define HEXA_RW_BY 8
 #endif
-#ifndef HEXA_USE_DMA
-#define HEXA_USE_DMA 1
-#endif
-#ifndef HEXA_DMA_BY
-#define HEXA_DMA_BY 8
-#endif
 constexpr int WL = 64;           // lanes per wavefront = node columns per tile
 constexpr int OWN_X = WL - 2;    // owned node columns per tile
 constexpr int BY = HEXA_RW_BY;   // wavefronts per block = node rows per tile
@@ -449,326 +443,6 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
     }
 }
 
-// ---------------------------------------------------------------------------
-// The same operator with its node planes and element factors brought into
-// LDS by the memory system itself (buffer loads to LDS): D planes ahead are
-// in flight without holding registers, and the node values a layer needs
-// (this row and the row above, this node and the +x neighbour) are read from
-// LDS, so the x-y modes need no lane shifts.
-//
-// Per block: BY waves; wave w owns element row w; node rows 0..BY of the
-// tile are staged (BY+1 rows of 64 nodes, packed as in memory).  Tile tx
-// covers nodes 62 tx .. 62 tx + 63: tile 0 owns lanes 0..62, the others lanes
-// 1..62 (lane 0 is the halo node shared with the tile to the left), so a row
-// segment never starts before its row and every 16-byte chunk of it is read
-// from its own plane.  Lanes past the row end hold the next row's values (or
-// zeros past the plane end); elements there have their factor forced to
-// zero, and the row's last node takes its +x neighbour as zero.
-//
-// Ordering: the loads to LDS are counted by vmcnt like any other vector
-// memory access, in issue order.  Every layer issues at least MIN_OPS
-// vector memory operations (its staging loads, its output store, the flags
-// of a later output plane), so before the barrier of layer kl, waiting until
-// at most (D - 2) * MIN_OPS operations are outstanding completes the
-// staging loads issued D - 2 layers earlier (plane kl + 2, read after the
-// barrier).  A plane's rows are read right after the barrier two layers
-// below it and its factors by its own layer, so a buffer restaged D + 1
-// layers after it was staged is free once every wave has passed that
-// layer's barrier.  Used for the FP32 sweeps (APPLY, RESID, SMOOTH).
-template <typename T>
-struct DmaGeom {
-    static constexpr int D = sizeof(T) == 8 ? 5 : 6;           // planes staged ahead of the layer
-    static constexpr int NB = D + 1;                            // plane buffers
-    static constexpr int BY = HEXA_DMA_BY;                     // waves per block
-    static constexpr int RCH = WL * 3 * (int)sizeof(T) / 16;  // 16-byte chunks per node row
-    static constexpr int NC = (BY + 1) * RCH;                   // chunks per staged plane
-    static constexpr int ROUNDS = (NC + WL * BY - 1) / (WL * BY);
-    static constexpr int PBYTES = (NC + WL - 1) / WL * WL * 16; // whole waves' worth
-    static constexpr int FCH = WL * (int)sizeof(T) / 16;        // chunks per element row
-    static constexpr int NFC = BY * FCH;
-    static constexpr int FBYTES = (NFC + WL - 1) / WL * WL * 16;
-    static constexpr int MIN_OPS = ROUNDS + 3;
-};
-
-// s_waitcnt immediate (gfx9 layout): vmcnt <= n, lgkmcnt 0, expcnt not waited
-constexpr int waitcnt_vm_lgkm0(int n) { return (n & 0xF) | (((n >> 4) & 3) << 14) | (7 << 4); }
-
-__device__ __forceinline__ void lds_dma16(Rsrc r, void* lds, uint32_t voff) {
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, voff, 0, 0, 0);
-}
-
-template <typename T, typename DT, int MODE>
-__global__ __launch_bounds__(WL* DmaGeom<T>::BY) void k_elastic_dma(
-    Grid3 g, FactCoef<T> coef, const T* __restrict__ fac, const uint8_t* __restrict__ mask,
-    const T* __restrict__ u, const T* __restrict__ b, const DT* __restrict__ dinv, T omega, T shift,
-    T* __restrict__ y, int kchunk, int tiles_x, int tiles_y, int tiles, double* __restrict__ dot_part) {
-    using G = DmaGeom<T>;
-    constexpr int BY = G::BY, OWN_Y = BY - 1;
-    __shared__ __attribute__((aligned(16))) char lds_u[G::NB][G::PBYTES];
-    __shared__ __attribute__((aligned(16))) char lds_f[G::NB][G::FBYTES];
-    __shared__ __attribute__((aligned(16))) char lds_dummy[WL * 16];
-    __shared__ T sc[2][BY][3][WL];   // each wave's results for the row above it
-
-    const int l = threadIdx.x;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    T cl[3][2], cm[3][2];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            cl[d][q] = in_vgpr(coef.cl[d][q]);
-            cm[d][q] = in_vgpr(coef.cm[d][q]);
-        }
-    const int L = blockIdx.x;
-    const int xcd = L % NXCD, qx = tiles / NXCD, rx = tiles % NXCD;
-    const int t = xcd * qx + min(xcd, rx) + L / NXCD;
-    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
-
-    const int i0 = tx * OWN_X;
-    const int i = i0 + l;
-    const int jt = ty * OWN_Y - 1;   // node row of wave 0
-    const int j = jt + w;
-    const int kb = tz * kchunk;
-    const int ke = min(kb + kchunk, g.nz + 1);
-    const bool col_in = (unsigned)i <= (unsigned)g.nx && (unsigned)j <= (unsigned)g.ny;
-    const bool elem = l < WL - 1 && (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-    const bool owner = l <= WL - 2 && (tx == 0 || l >= 1) && w >= 1 && col_in;
-    const int64_t plane_n = (int64_t)(g.nx + 1) * (g.ny + 1);
-    const int64_t plane_e = (int64_t)g.nx * g.ny;
-    const int64_t vplane = 3 * plane_n;
-    const int64_t nrow = i + (int64_t)(g.nx + 1) * j;
-    const uint32_t ooff = owner ? (uint32_t)(nrow * 3 * sizeof(T)) : OOB;
-    const uint32_t moff = owner ? (uint32_t)nrow : OOB;
-    const uint32_t doff = owner ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
-
-    // staging addresses (loop-invariant): chunk c of the tile plane goes to
-    // LDS byte 16 c; a wave whose chunks all lie past the plane writes to a
-    // dummy area
-    uint32_t uvo[G::ROUNDS];
-    int uld[G::ROUNDS];
-#pragma unroll
-    for (int q = 0; q < G::ROUNDS; ++q) {
-        const int c = WL * (BY * q + w) + l;
-        const int row = c / G::RCH, off = (c % G::RCH) * 16;
-        const int jr = jt + row;
-        const bool ok = c < G::NC && (unsigned)jr <= (unsigned)g.ny;
-        uvo[q] = ok ? (uint32_t)((i0 + (int64_t)(g.nx + 1) * jr) * 3 * sizeof(T) + off) : OOB;
-        uld[q] = WL * (BY * q + w) < G::NC ? WL * 16 * (BY * q + w) : -1;
-    }
-    uint32_t fvo;
-    int fld;
-    {
-        const int c = WL * w + l;
-        const int row = c / G::FCH, off = (c % G::FCH) * 16;
-        const int jr = jt + row;
-        const bool ok = c < G::NFC && (unsigned)jr < (unsigned)g.ny;
-        fvo = ok ? (uint32_t)((i0 + (int64_t)g.nx * jr) * sizeof(T) + off) : OOB;
-        fld = WL * w < G::NFC ? WL * 16 * w : -1;
-    }
-    auto buf = [&](int k) { return (k - kb + 1) % G::NB; };
-    auto stage = [&](int k) {
-        const bool in = (unsigned)k <= (unsigned)g.nz;
-        const Rsrc ru = rsrc(u + (in ? vplane * k : 0), in ? vplane * (int64_t)sizeof(T) : 0);
-        const int bk = buf(k);
-#pragma unroll
-        for (int q = 0; q < G::ROUNDS; ++q) lds_dma16(ru, uld[q] >= 0 ? &lds_u[bk][uld[q]] : lds_dummy, uvo[q]);
-        const bool fin = (unsigned)k < (unsigned)g.nz;
-        const Rsrc rf = rsrc(fac + (fin ? plane_e * k : 0), fin ? plane_e * (int64_t)sizeof(T) : 0);
-        lds_dma16(rf, fld >= 0 ? &lds_f[bk][fld] : lds_dummy, fvo);
-    };
-    // node (row r of the tile, lane) of a staged plane
-    const int lx = l < WL - 1 ? l + 1 : l;
-    auto node = [&](int k, int r, int lane, T (&o)[3]) {
-        const T* p = reinterpret_cast<const T*>(lds_u[buf(k)]) + (r * WL + lane) * 3;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) o[c] = p[c];
-    };
-    // the four nodes of this lane's element column in plane k: own, +x,
-    // +y, +x+y
-    struct Quad {
-        T a[3], bx[3], cy[3], d[3];
-    };
-    // (the +x neighbours of the last node of a row are values of the next
-    // row; they meet only a zero element factor, but are zeroed so that an
-    // Inf or NaN there cannot reach the row's last node)
-    const bool xin = i < g.nx;
-    auto fetch = [&](int k, Quad& n) {
-        node(k, w, l, n.a);
-        node(k, w, lx, n.bx);
-        node(k, w + 1, l, n.cy);
-        node(k, w + 1, lx, n.d);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            n.bx[c] = xin ? n.bx[c] : T(0);
-            n.d[c] = xin ? n.d[c] : T(0);
-        }
-    };
-    // x-y modes of a fetched plane, and this lane's own node values
-    auto plane_modes = [&](const Quad& n, T (&P)[3][4], T (&own)[3]) {
-        const T(&a)[3] = n.a;
-        const T(&bx)[3] = n.bx;
-        const T(&cy)[3] = n.cy;
-        const T(&d)[3] = n.d;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) own[c] = a[c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T s0 = a[c] + bx[c], d0 = bx[c] - a[c], s1 = cy[c] + d[c], d1 = d[c] - cy[c];
-            P[c][0] = s0 + s1;
-            P[c][1] = d0 + d1;
-            P[c][2] = s1 - s0;
-            P[c][3] = d1 - d0;
-        }
-    };
-    struct Out {
-        Rsrc ry;
-        uint32_t mb;
-        bool has_out;
-        T bv[3], dv[3];
-    };
-    double acc = 0.0;
-    auto load_out = [&](const int kl, Out& o) {
-        const bool has_out = kl >= kb && kl < ke;
-        const int64_t ko = has_out ? kl : 0;
-        o.has_out = has_out;
-        o.ry = rsrc(y + vplane * ko, has_out ? vplane * (int64_t)sizeof(T) : 0);
-        o.mb = __builtin_amdgcn_raw_buffer_load_b8(
-            rsrc(mask ? mask + plane_n * ko : mask, (mask && has_out) ? plane_n : 0), moff, 0, 0);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) o.bv[c] = o.dv[c] = T(0);
-        if (MODE != OP_APPLY) ld3(rsrc(b + vplane * ko, has_out ? vplane * (int64_t)sizeof(T) : 0), ooff, o.bv);
-        if (MODE == OP_SMOOTH)
-            ld3_dinv(rsrc(dinv + vplane * ko, has_out ? vplane * (int64_t)sizeof(DT) : 0), doff, dinv, o.dv);
-    };
-    // wait for the staging loads of D - 2 layers ago and for this wave's LDS
-    // writes, then the block barrier
-    constexpr int NW = (G::D - 2) * G::MIN_OPS;
-    static_assert(NW < 64, "vmcnt range");
-    auto sync = [&]() {
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(NW));
-        __builtin_amdgcn_s_barrier();
-        asm volatile("" ::: "memory");
-    };
-
-    // qn holds plane kl+1's nodes on entry and plane kl+2's on return (read
-    // right after the barrier, so their latency hides behind the output)
-    auto step = [&](auto par, const int kl, const T (&Plo)[3][4], T (&Phi)[3][4], const T (&Yin)[3][4],
-                    T (&Yout)[3][4], const T (&ulo)[3], T (&uhi)[3], Quad& qn, Out& ot) {
-        constexpr int P = decltype(par)::value;
-        stage(kl + G::D);
-        asm volatile("" ::: "memory");
-        plane_modes(qn, Phi, uhi);
-        const T fl = elem ? reinterpret_cast<const T*>(lds_f[buf(kl)])[w * WL + l] : T(0);
-        T N0[3];
-        {
-            T v[3][8];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    if (m) v[c][m] = Plo[c][m] + Phi[c][m];
-                    v[c][m + 4] = Phi[c][m] - Plo[c][m];
-                }
-            T N1[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                T yv[8];
-#pragma unroll
-                for (int m = 1; m < 8; ++m) yv[m] = coef.D[c][m] * v[c][m];
-#pragma unroll
-                for (int bb = 0; bb < 3; ++bb) {
-                    if (bb == c) continue;
-                    const int d = 3 - c - bb;
-#pragma unroll
-                    for (int qq = 0; qq < 2; ++qq) {
-                        const int mq = qq << d;
-                        yv[(1 << c) | mq] += cl[d][qq] * v[bb][(1 << bb) | mq];
-                        yv[(1 << bb) | mq] += cm[d][qq] * v[bb][(1 << c) | mq];
-                    }
-                }
-                T zlo[4];
-                Yout[c][0] = fl * yv[4];
-                zlo[0] = Yin[c][0] - fl * yv[4];
-#pragma unroll
-                for (int m = 1; m < 4; ++m) {
-                    Yout[c][m] = fl * (yv[m] + yv[m + 4]);
-                    zlo[m] = Yin[c][m] + fl * (yv[m] - yv[m + 4]);
-                }
-                const T t0 = zlo[0] - zlo[1], t1 = zlo[0] + zlo[1], t2 = zlo[2] - zlo[3], t3 = zlo[2] + zlo[3];
-                const T c00 = t0 - t2, c10 = t1 - t3, c01 = t0 + t2, c11 = t1 + t3;
-                N0[c] = c00 + from_prev(c10);
-                N1[c] = c01 + from_prev(c11);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sc[P][w][c][l] = N1[c];
-        }
-        sync();
-        fetch(kl + 2, qn);
-        T o[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T below = sc[P][w > 0 ? w - 1 : 0][c][l];
-            const T Au = N0[c] + (owner ? below : T(0)) + shift * ulo[c];
-            T v;
-            if (MODE == OP_APPLY) v = Au;
-            else if (MODE == OP_RESID) v = ot.bv[c] - Au;
-            else v = ulo[c] + omega * ot.dv[c] * (ot.bv[c] - Au);
-            o[c] = ((ot.mb >> c) & 1) ? T(0) : v;
-        }
-        st3(ot.ry, ooff, o);
-        if (dot_part && ot.has_out && owner) acc += (double)ulo[0] * o[0] + (double)ulo[1] * o[1] + (double)ulo[2] * o[2];
-        load_out(kl + 2, ot);
-        asm volatile("" ::: "memory");
-    };
-    using P0 = std::integral_constant<int, 0>;
-    using P1 = std::integral_constant<int, 1>;
-
-    T PA[3][4], PB[3][4], YA[3][4], YB[3][4], UA[3], UB[3];
-#pragma unroll
-    for (int k = 0; k < G::D; ++k) stage(kb - 1 + k);
-    Out o0, o1;
-    load_out(kb - 1, o0);
-    load_out(kb, o1);
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(0));
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-    Quad qn;
-    fetch(kb - 1, qn);
-    plane_modes(qn, PA, UA);
-    fetch(kb, qn);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) YA[c][m] = T(0);
-
-    // layers in pairs (a pair's second layer past ke has no output): a
-    // branch between the two would make the waits above the barriers
-    // conservative
-    for (int kl = kb - 1; kl < ke; kl += 2) {
-        step(P0{}, kl, PA, PB, YA, YB, UA, UB, qn, o0);
-        step(P1{}, kl + 1, PB, PA, YB, YA, UB, UA, qn, o1);
-    }
-    // no staging load may still be writing LDS when the block ends
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(0));
-    asm volatile("" ::: "memory");
-    if (dot_part) {
-        double v = acc;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        __syncthreads();
-        double* red = reinterpret_cast<double*>(&sc[0][0][0][0]);
-        if (l == 0) red[w] = v;
-        __syncthreads();
-        if (l == 0 && w == 0) {
-            double tsum = 0.0;
-            for (int q = 0; q < BY; ++q) tsum += red[q];
-            dot_part[L] = tsum;
-        }
-    }
-}
-
 }  // namespace
 
 namespace {
@@ -779,7 +453,7 @@ struct RwShape {
 // Chunk lengths (node planes per block along z) measured fastest for a grid
 // and kernel (tune_elastic_rw), shared by all handles of the process; grids
 // not tuned use the block-count rule below.
-enum TuneKind { TK_RW64 = 0, TK_RW32 = 1, TK_DMA32 = 2 };
+enum TuneKind { TK_RW64 = 0, TK_RW32 = 1 };
 std::mutex g_tune_mu;
 std::map<std::array<int, 4>, int> g_tuned;
 int tuned_chunk(const Grid3& g, int kind) {
@@ -809,28 +483,10 @@ bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
     return 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
 }
 
-// tiling of k_elastic_dma: tile tx starts at node 62 tx; odd chunks of
-// node planes give an even number of element layers per block
-RwShape dma_shape(const Grid3& g, int by, int kchunk = 0) {
-    RwShape r;
-    r.tiles_x = std::max(1, (g.nx + OWN_X - 1) / OWN_X);
-    r.tiles_y = (g.ny + 1 + by - 2) / (by - 1);
-    r.kchunk = kchunk ? kchunk : tuned_chunk(g, TK_DMA32);
-    if (!r.kchunk) {
-        r.kchunk = 31;
-        while (r.kchunk > 3 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
-            r.kchunk /= 2;
-    }
-    r.tiles_z = (g.nz + 1 + r.kchunk - 1) / r.kchunk;
-    return r;
-}
-
-// (the shortest chunks any tiling uses: the most blocks)
+// (the shortest chunks the tuning tries: the most blocks)
 int64_t elastic_rw_blocks(const Grid3& g) {
-    const RwShape r = rw_shape(g, TK_RW64, 4), a = dma_shape(g, DmaGeom<double>::BY, 3),
-                  f = dma_shape(g, DmaGeom<float>::BY, 3);
-    return std::max({(int64_t)r.tiles_x * r.tiles_y * r.tiles_z, (int64_t)a.tiles_x * a.tiles_y * a.tiles_z,
-                     (int64_t)f.tiles_x * f.tiles_y * f.tiles_z});
+    const RwShape r = rw_shape(g, TK_RW64, 4);
+    return (int64_t)r.tiles_x * r.tiles_y * r.tiles_z;
 }
 
 template <typename T, typename DT>
@@ -842,26 +498,6 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
         launch_elastic_fact<T, DT>(s, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
         return 0;
     }
-#if HEXA_USE_DMA
-    // FP32 sweeps stage through LDS; the FP64 operator, one block per CU at
-    // its register count, measured faster with its register prefetch ring
-    if constexpr (sizeof(T) == 4) if (mode != OP_SMOOTH0) {
-        const RwShape r = dma_shape(g, DmaGeom<T>::BY);
-        const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
-        const dim3 block(WL, DmaGeom<T>::BY);
-        if (mode == OP_APPLY)
-            k_elastic_dma<T, DT, OP_APPLY><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
-        else if (mode == OP_RESID)
-            k_elastic_dma<T, DT, OP_RESID><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
-        else
-            k_elastic_dma<T, DT, OP_SMOOTH><<<tiles, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift,
-                                                                   y, r.kchunk, r.tiles_x, r.tiles_y, tiles,
-                                                                   dot_part);
-        return dot_part ? tiles : 0;
-    }
-#endif
     const RwShape r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32);
     const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
     dim3 block(WL, BY);
@@ -887,9 +523,8 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
 }
 
 // Times K(rho)u (APPLY) for each candidate chunk length on this grid and
-// records the fastest for the kernels of precision T (FP64: the register
-// kernel; FP32: the staged kernel and the register kernel of the first
-// smoothing sweep).  u is read and y overwritten.  Grids already tuned, too
+// records the fastest for the kernel of precision T.  u is read and y
+// overwritten.  Grids already tuned, too
 // small to matter, or HEXA_TUNE=0 keep the block-count rule.  The chunk
 // length changes the order of the per-block (u, Au) partial sums, so results
 // can differ in the last bits between a tuned and an untuned run.
@@ -897,50 +532,42 @@ template <typename T>
 void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, const T* fac, const T* u, T* y) {
     const char* te = getenv("HEXA_TUNE");
     if ((te && te[0] == '0') || (int64_t)g.nx * g.ny * g.nz < 32768 || !elastic_rw_supported(g, sizeof(T))) return;
-    const int kinds[2] = {sizeof(T) == 8 ? (int)TK_RW64 : (int)TK_DMA32, (int)TK_RW32};
+    const int kind = sizeof(T) == 8 ? (int)TK_RW64 : (int)TK_RW32;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return;
     if (hipEventCreate(&e1) != hipSuccess) {
         hipEventDestroy(e0);
         return;
     }
-    for (int kk = 0; kk < (sizeof(T) == 8 ? 1 : 2); ++kk) {
-        const int kind = kinds[kk];
-        if (tuned_chunk(g, kind)) continue;
-        const bool dma = kind == TK_DMA32;
-        int best = 0;
-        float best_ms = 0.0f;
-        for (const int kc : dma ? std::array<int, 4>{31, 15, 7, 3} : std::array<int, 4>{32, 16, 8, 4}) {
-            const RwShape r = dma ? dma_shape(g, DmaGeom<float>::BY, kc) : rw_shape(g, kind, kc);
+    if (!tuned_chunk(g, kind)) {
+        const int cand[4] = {32, 16, 8, 4};
+        float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        auto launch = [&](int kc) {
+            const RwShape r = rw_shape(g, kind, kc);
             const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
-            auto launch = [&]() {
-                if constexpr (sizeof(T) == 4) {
-                    if (dma) {
-                        k_elastic_dma<T, T, OP_APPLY><<<tiles, dim3(WL, DmaGeom<T>::BY), 0, s>>>(
-                            g, coef, fac, nullptr, u, nullptr, nullptr, T(0), T(0), y, r.kchunk, r.tiles_x,
-                            r.tiles_y, tiles, nullptr);
-                        return;
-                    }
-                }
-                k_elastic_rw<T, T, OP_APPLY><<<tiles, dim3(WL, BY), 0, s>>>(g, coef, fac, nullptr, u, nullptr,
-                                                                            nullptr, T(0), T(0), y, r.kchunk,
-                                                                            r.tiles_x, r.tiles_y, tiles, nullptr);
-            };
-            launch();
-            launch();
-            hipEventRecord(e0, s);
-            for (int rep = 0; rep < 3; ++rep) launch();
-            hipEventRecord(e1, s);
-            float ms = 0.0f;
-            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                best = 0;
-                break;
+            k_elastic_rw<T, T, OP_APPLY><<<tiles, dim3(WL, BY), 0, s>>>(g, coef, fac, nullptr, u, nullptr, nullptr,
+                                                                        T(0), T(0), y, r.kchunk, r.tiles_x,
+                                                                        r.tiles_y, tiles, nullptr);
+        };
+        // the GPU first reaches its sustained clocks, then every candidate is
+        // timed twice (ascending, then descending) and keeps its better time
+        for (int w = 0; w < 8; ++w) launch(cand[0]);
+        bool ok = true;
+        for (int pass = 0; pass < 2 && ok; ++pass)
+            for (int q = 0; q < 4 && ok; ++q) {
+                const int c = pass ? 3 - q : q;
+                launch(cand[c]);
+                hipEventRecord(e0, s);
+                for (int rep = 0; rep < 3; ++rep) launch(cand[c]);
+                hipEventRecord(e1, s);
+                float ms = 0.0f;
+                ok = hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                if (ok && (pass == 0 || ms < t[c])) t[c] = ms;
             }
-            if (!best || ms < best_ms) {
-                best = kc;
-                best_ms = ms;
-            }
-        }
+        int best = 0;
+        for (int c = 0; ok && c < 4; ++c)
+            if (!best || t[c] < t[best - 1]) best = c + 1;
+        best = ok && best ? cand[best - 1] : 0;
         if (best) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tuned[{g.nx, g.ny, g.nz, kind}] = best;
