@@ -84,45 +84,66 @@ __global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, FastDiv dnx, 
     st(rc, n, V3<T>{(mb & 1) ? T(0) : s.x, (mb & 2) ? T(0) : s.y, (mb & 4) ? T(0) : s.z});
 }
 
-// x_f += P x_c, masked by the fine supports; one thread per fine node.
-// Along each axis an even fine index coincides with coarse node i/2 (weight
-// 1), an odd one lies midway between (i-1)/2 and (i+1)/2 (weights 1/2).
-// Coarse nodes beyond the coarse grid (the last fine node of an odd axis
-// under a floor(n/2) coarse grid) contribute nothing.
+// x_f += P x_c, masked by the fine supports.  One thread per pair of fine
+// nodes (2m, 2m+1) of a row: the even node coincides with coarse node m
+// (weight 1 along x), the odd one lies midway between m and m+1 (weights
+// 1/2), so the pair reads each coarse column once.  Along y and z an even
+// fine index takes coarse row/plane j/2, an odd one the two around it (1/2
+// each).  Coarse nodes beyond the coarse grid (the last fine node of an odd
+// axis under a floor(n/2) coarse grid) contribute nothing.
 template <typename T>
-__global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, FastDiv dnx, FastDiv dny, uint32_t nfn,
+__global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, FastDiv dpr, FastDiv dny, uint32_t npairs,
                                                     const T* __restrict__ xc, const uint8_t* __restrict__ fmask,
                                                     T* __restrict__ xf) {
-    const uint32_t n = blockIdx.x * TB + threadIdx.x;
-    if (n >= nfn) return;
-    const uint8_t mb = fmask[n];
-    if ((mb & 7) == 7) return;
-    const uint32_t row = n / dnx, k = row / dny;
-    const int i = (int)(n - row * dnx.d), j = (int)(row - k * dny.d);
-    const int CNX = c.nx + 1, CNY = c.ny + 1, CNZ = c.nz + 1;
-    const int i0 = i >> 1, j0 = j >> 1, k0 = (int)k >> 1;
-    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-    V3<T> s{T(0), T(0), T(0)};
+    const uint32_t q = blockIdx.x * TB + threadIdx.x;
+    if (q >= npairs) return;
+    const uint32_t row = q / dpr, k = row / dny;
+    const int m = (int)(q - row * dpr.d), j = (int)(row - k * dny.d);
+    const int FNX = f.nx + 1, CNX = c.nx + 1, CNY = c.ny + 1, CNZ = c.nz + 1;
+    const int i = 2 * m;
+    const bool odd_in = i + 1 < FNX;                      // the pair's odd node exists
+    const int64_t n = (int64_t)row * FNX + i;
+    const uint8_t mb0 = fmask[n], mb1 = odd_in ? fmask[n + 1] : (uint8_t)7;
+    if ((mb0 & 7) == 7 && (mb1 & 7) == 7) return;
+    const int j0 = j >> 1, k0 = (int)k >> 1;
+    const int nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    const bool c1_in = m + 1 < CNX;                       // coarse column m+1 exists
+    V3<T> s0{T(0), T(0), T(0)}, s1{T(0), T(0), T(0)};    // coarse column m, m+1 sums
     for (int a = 0; a < nk; ++a) {
         if (k0 + a >= CNZ) continue;
         for (int bb = 0; bb < nj; ++bb) {
             if (j0 + bb >= CNY) continue;
             const int64_t crow = (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
-            for (int cc = 0; cc < ni; ++cc) {
-                if (i0 + cc >= CNX) continue;
-                const V3<T> v = ld(xc, crow + i0 + cc);
-                s.x += v.x;
-                s.y += v.y;
-                s.z += v.z;
+            if (m < CNX) {
+                const V3<T> v = ld(xc, crow + m);
+                s0.x += v.x;
+                s0.y += v.y;
+                s0.z += v.z;
+            }
+            if (c1_in) {
+                const V3<T> v = ld(xc, crow + m + 1);
+                s1.x += v.x;
+                s1.y += v.y;
+                s1.z += v.z;
             }
         }
     }
-    const T w = (ni == 2 ? T(0.5) : T(1)) * (nj == 2 ? T(0.5) : T(1)) * (nk == 2 ? T(0.5) : T(1));
-    V3<T> x = ld(xf, n);
-    if (!(mb & 1)) x.x += w * s.x;
-    if (!(mb & 2)) x.y += w * s.y;
-    if (!(mb & 4)) x.z += w * s.z;
-    st(xf, n, x);
+    const T wyz = (nj == 2 ? T(0.5) : T(1)) * (nk == 2 ? T(0.5) : T(1));
+    if ((mb0 & 7) != 7) {
+        V3<T> x = ld(xf, n);
+        if (!(mb0 & 1)) x.x += wyz * s0.x;
+        if (!(mb0 & 2)) x.y += wyz * s0.y;
+        if (!(mb0 & 4)) x.z += wyz * s0.z;
+        st(xf, n, x);
+    }
+    if ((mb1 & 7) != 7) {
+        const T w = T(0.5) * wyz;
+        V3<T> x = ld(xf, n + 1);
+        if (!(mb1 & 1)) x.x += w * (s0.x + s1.x);
+        if (!(mb1 & 2)) x.y += w * (s0.y + s1.y);
+        if (!(mb1 & 4)) x.z += w * (s0.z + s1.z);
+        st(xf, n + 1, x);
+    }
 }
 
 // Reference 2-level preconditioner, coarse diagonal
@@ -260,9 +281,10 @@ void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, cons
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
                         const uint8_t* fmask, T* xf) {
-    const uint32_t nfn = (uint32_t)fine.nn;
-    k_prolong_add<T><<<(nfn + TB - 1) / TB, TB, 0, s>>>(fine, coarse, fast_div(fine.nx + 1), fast_div(fine.ny + 1),
-                                                        nfn, xc, fmask, xf);
+    const uint32_t ppr = (uint32_t)(fine.nx + 2) / 2;   // node pairs per fine row
+    const uint32_t npairs = ppr * (uint32_t)((fine.ny + 1) * (fine.nz + 1));
+    k_prolong_add<T><<<(npairs + TB - 1) / TB, TB, 0, s>>>(fine, coarse, fast_div(ppr), fast_div(fine.ny + 1), npairs,
+                                                           xc, fmask, xf);
 }
 
 void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
