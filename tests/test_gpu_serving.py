@@ -34,7 +34,7 @@ def test_result_same_alone_and_in_mix(precision):
         ids += [s.submit(precision=precision, **j) for j in OTHERS[2:]]
         rb, xb = s.wait(ida, want_x=True)
         rest = [s.wait(i) for i in ids]
-    assert all(r["status"] == 0 for r in rest)
+    assert all(r["status"] == 0 for r in rest + [rb]), [(r["status"], r.get("error")) for r in rest + [rb]]
     assert np.array_equal(xa, xb)
     assert ra["compliance"] == rb["compliance"] and ra["solve_iterations"] == rb["solve_iterations"]
     # the mix really overlapped on the device
@@ -94,3 +94,22 @@ def test_served_job_matches_direct_oc_loop():
                                  warm_start=int(it > 0))
     assert np.array_equal(x, xd)
     assert r["compliance"] == rep["compliance"]
+
+
+def test_concurrent_captures_and_allocations():
+    """Many short jobs on 8 workers: graph captures (the first PCG and filter
+    solve of every job) overlap other workers' handle creation, allocation
+    and teardown.  Every job succeeds, and each equals its run alone."""
+    say("concurrent captures")
+    jobs = [dict(nx=8 + 4 * (i % 4), ny=8, nz=4 + 4 * (i % 3), support_face=i % 6, n_loads=1 + i % 2, oc_iters=2,
+                 volfrac=0.3, seed=200 + i) for i in range(16)]
+    with hexa.Server(workers=8) as s:
+        ids = [s.submit(precision=(hexa.PREC_MIXED if i % 2 else hexa.PREC_FP64), **j) for i, j in enumerate(jobs)]
+        res = [s.wait(i, want_x=True) for i in ids]
+    bad = [(r["status"], r.get("error")) for r, _ in res if r["status"] != 0]
+    assert not bad, bad
+    with hexa.Server(workers=1) as s:
+        for i in (0, 5, 10, 15):
+            r, x = s.wait(s.submit(precision=(hexa.PREC_MIXED if i % 2 else hexa.PREC_FP64), **jobs[i]), want_x=True)
+            assert r["status"] == 0
+            assert np.array_equal(x, res[i][1]) and r["compliance"] == res[i][0]["compliance"]

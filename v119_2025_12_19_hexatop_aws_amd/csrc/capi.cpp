@@ -35,6 +35,11 @@ void set_error(const char* fmt, ...) {
     g_err = buf;
 }
 
+std::shared_mutex& capture_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+
 }  // namespace hexa
 
 using namespace hexa;
@@ -43,7 +48,11 @@ namespace hexa {
 
 int32_t dalloc_bytes(hexa_handle* h, void** p, size_t bytes) {
     if (*p) return HEXA_OK;
-    hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+    hipError_t e;
+    {
+        DeviceOpGuard lock;
+        e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
@@ -441,6 +450,7 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
     const bool graphs = !(ge && ge[0] == '0');
     if (graphs && !(h->pcg_exec && h->pcg_key.pk == pk && h->pcg_key.cpu == (int)cpu &&
                     h->pcg_key.every == check_every && h->pcg_key.s == s && h->pcg_key.gen == h->graph_gen)) {
+        CaptureGuard lock;
         if (h->pcg_exec) {
             HX_CHECK(hipGraphExecDestroy(h->pcg_exec));
             h->pcg_exec = nullptr;
@@ -805,15 +815,26 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
     h->coef32 = make_fact_coef<float>(grid->dx, grid->dy, grid->dz, o.nu);
     auto fail = [&](int32_t s) { hexa_destroy(h); return s; };
     if (hipGetDevice(&h->device) != hipSuccess) return fail(HEXA_E_DEVICE);
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(HEXA_E_DEVICE);
-    if (hipEventCreate(&h->ev_a) != hipSuccess || hipEventCreate(&h->ev_b) != hipSuccess) return fail(HEXA_E_DEVICE);
-    for (int i = 0; i < RING; ++i)
-        if (hipEventCreateWithFlags(&h->ring_ev[i], hipEventDisableTiming) != hipSuccess) return fail(HEXA_E_DEVICE);
-    if (hipHostMalloc((void**)&h->h_state, sizeof(PcgState) * RING, hipHostMallocDefault) != hipSuccess)
+    bool ok;
+    {
+        DeviceOpGuard lock;
+        ok = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreate(&h->ev_a) == hipSuccess && hipEventCreate(&h->ev_b) == hipSuccess &&
+             hipHostMalloc((void**)&h->h_state, sizeof(PcgState) * RING, hipHostMallocDefault) == hipSuccess;
+        for (int i = 0; ok && i < RING; ++i)
+            ok = hipEventCreateWithFlags(&h->ring_ev[i], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+        set_error("stream, event or pinned buffer creation failed");
         return fail(HEXA_E_DEVICE);
+    }
     int32_t s = dalloc(h, &h->d_Ke64, 576);
     if (s != HEXA_OK) return fail(s);
-    if (hipMemcpy(h->d_Ke64, h->Ke, sizeof h->Ke, hipMemcpyHostToDevice) != hipSuccess) return fail(HEXA_E_DEVICE);
+    if (hipMemcpyAsync(h->d_Ke64, h->Ke, sizeof h->Ke, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        set_error("element matrix upload failed");
+        return fail(HEXA_E_DEVICE);
+    }
     *out = h;
     return HEXA_OK;
 }
@@ -821,6 +842,7 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
 int32_t hexa_destroy(hexa_handle* h) {
     if (!h) return HEXA_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    DeviceOpGuard lock;
     if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
     if (h->filt_exec) hipGraphExecDestroy(h->filt_exec);
     for (void* p : h->allocs) hipFree(p);

@@ -73,6 +73,7 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     const char* ge = getenv("HEXA_GRAPHS");
     const bool graphs = !(ge && ge[0] == '0');
     if (graphs && !(h->filt_exec && h->filt_every == check_every)) {
+        CaptureGuard lock;
         if (h->filt_exec) {
             HX_CHECK(hipGraphExecDestroy(h->filt_exec));
             h->filt_exec = nullptr;

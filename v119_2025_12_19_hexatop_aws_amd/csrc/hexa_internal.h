@@ -6,12 +6,30 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
+#include <shared_mutex>
 
 #include "../../include/hexa.h"
 
 namespace hexa {
 
 void set_error(const char* fmt, ...);
+
+// A stream capture fails, and so does the other thread's call, when another
+// thread of the process meanwhile makes a HIP call that is not ordered on a
+// stream of its own (allocation and freeing, stream, event and graph
+// creation and destruction, synchronous copies, launches on the legacy
+// stream).  Every such call in the library holds this lock shared; a capture
+// holds it exclusively from the destruction of the graph it replaces to the
+// instantiation of the new one.  Neither is taken while the other is held
+// by the same thread.
+std::shared_mutex& capture_mutex();
+struct CaptureGuard {
+    std::unique_lock<std::shared_mutex> lk{capture_mutex()};
+};
+struct DeviceOpGuard {
+    std::shared_lock<std::shared_mutex> lk{capture_mutex()};
+};
 
 #define HX_CHECK(call)                                                                  \
     do {                                                                                \

@@ -263,24 +263,39 @@ int32_t hexa_regrid_density_host(const hexa_grid* old_grid, const float* old_den
     const size_t no = (size_t)old_grid->nx * old_grid->ny * old_grid->nz;
     const size_t nn = (size_t)new_grid->nx * new_grid->ny * new_grid->nz;
     float *d_o = nullptr, *d_oa = nullptr, *d_n = nullptr, *d_na = nullptr;
+    hipStream_t s = nullptr;   // a stream of its own: nothing here touches the legacy stream
     int32_t st = HEXA_OK;
-    auto alloc = [&](float** p, size_t n) {
-        if (st == HEXA_OK && hipMalloc((void**)p, sizeof(float) * (n ? n : 1)) != hipSuccess) {
-            set_error("device allocation failed");
-            st = HEXA_E_OOM;
+    {
+        DeviceOpGuard lock;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            set_error("stream creation failed");
+            st = HEXA_E_DEVICE;
         }
+        auto alloc = [&](float** p, size_t n) {
+            if (st == HEXA_OK && hipMalloc((void**)p, sizeof(float) * (n ? n : 1)) != hipSuccess) {
+                set_error("device allocation failed");
+                st = HEXA_E_OOM;
+            }
+        };
+        alloc(&d_o, no);
+        alloc(&d_n, nn);
+        if (old_alpha && new_alpha) { alloc(&d_oa, no); alloc(&d_na, nn); }
+    }
+    auto copy = [&](void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+        if (st == HEXA_OK && hipMemcpyAsync(dst, src, bytes, kind, s) != hipSuccess) st = HEXA_E_DEVICE;
     };
-    alloc(&d_o, no);
-    alloc(&d_n, nn);
-    if (old_alpha && new_alpha) { alloc(&d_oa, no); alloc(&d_na, nn); }
-    if (st == HEXA_OK && hipMemcpy(d_o, old_density, sizeof(float) * no, hipMemcpyHostToDevice) != hipSuccess) st = HEXA_E_DEVICE;
-    if (st == HEXA_OK && d_oa && hipMemcpy(d_oa, old_alpha, sizeof(float) * no, hipMemcpyHostToDevice) != hipSuccess) st = HEXA_E_DEVICE;
+    copy(d_o, old_density, sizeof(float) * no, hipMemcpyHostToDevice);
+    if (d_oa) copy(d_oa, old_alpha, sizeof(float) * no, hipMemcpyHostToDevice);
     if (st == HEXA_OK)
-        st = hexa_regrid_density(old_grid, d_o, d_oa, new_grid, len_x, len_y, len_z, hard, d_n, d_na, nullptr);
-    if (st == HEXA_OK && hipMemcpy(new_density, d_n, sizeof(float) * nn, hipMemcpyDeviceToHost) != hipSuccess) st = HEXA_E_DEVICE;
-    if (st == HEXA_OK && d_na && hipMemcpy(new_alpha, d_na, sizeof(float) * nn, hipMemcpyDeviceToHost) != hipSuccess) st = HEXA_E_DEVICE;
+        st = hexa_regrid_density(old_grid, d_o, d_oa, new_grid, len_x, len_y, len_z, hard, d_n, d_na, s);
+    copy(new_density, d_n, sizeof(float) * nn, hipMemcpyDeviceToHost);
+    if (d_na) copy(new_alpha, d_na, sizeof(float) * nn, hipMemcpyDeviceToHost);
+    if (st == HEXA_OK && hipStreamSynchronize(s) != hipSuccess) st = HEXA_E_DEVICE;
     if (st == HEXA_E_DEVICE) set_error("device copy failed");
+    if (s) hipStreamSynchronize(s);
+    DeviceOpGuard lock;
     hipFree(d_o); hipFree(d_oa); hipFree(d_n); hipFree(d_na);
+    if (s) hipStreamDestroy(s);
     return st;
 }
 

@@ -534,10 +534,13 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
     if ((te && te[0] == '0') || (int64_t)g.nx * g.ny * g.nz < 32768 || !elastic_rw_supported(g, sizeof(T))) return;
     const int kind = sizeof(T) == 8 ? (int)TK_RW64 : (int)TK_RW32;
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return;
-    if (hipEventCreate(&e1) != hipSuccess) {
-        hipEventDestroy(e0);
-        return;
+    {
+        DeviceOpGuard lock;
+        if (hipEventCreate(&e0) != hipSuccess) return;
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return;
+        }
     }
     if (!tuned_chunk(g, kind)) {
         const int cand[4] = {32, 16, 8, 4};
@@ -557,9 +560,9 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
             for (int q = 0; q < 4 && ok; ++q) {
                 const int c = pass ? 3 - q : q;
                 launch(cand[c]);
-                hipEventRecord(e0, s);
+                (void)hipEventRecord(e0, s);
                 for (int rep = 0; rep < 3; ++rep) launch(cand[c]);
-                hipEventRecord(e1, s);
+                (void)hipEventRecord(e1, s);
                 float ms = 0.0f;
                 ok = hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
                 if (ok && (pass == 0 || ms < t[c])) t[c] = ms;
@@ -573,8 +576,9 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
             g_tuned[{g.nx, g.ny, g.nz, kind}] = best;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    DeviceOpGuard lock;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 template void tune_elastic_rw<double>(hipStream_t, const Grid3&, const FactCoef<double>&, const double*,
                                       const double*, double*);

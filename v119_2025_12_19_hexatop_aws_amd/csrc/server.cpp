@@ -18,6 +18,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -43,6 +44,7 @@ struct Job {
     int64_t est = 0;
     bool done = false;
     std::vector<double> x;   // final design
+    std::string err;         // the worker's error message when status != OK
 };
 
 }  // namespace
@@ -191,6 +193,7 @@ void worker(hexa_server* s) {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             jb->res.status = st;
+            if (st != HEXA_OK) jb->err = hexa_last_error();
             jb->res.t_end_ms = wall_ms(s);
             jb->done = true;
             s->in_use -= jb->est;
@@ -262,6 +265,8 @@ int32_t hexa_server_wait(hexa_server* s, int64_t id, hexa_job_result* res, doubl
     if (res) *res = jb->res;
     if (x_out) std::memcpy(x_out, jb->x.data(), sizeof(double) * jb->x.size());
     const int32_t st = jb->res.status;
+    // error messages are per thread: hand the worker's to the waiter
+    if (st != HEXA_OK) set_error("job %lld: %s", (long long)id, jb->err.c_str());
     delete jb;
     return st;
 }

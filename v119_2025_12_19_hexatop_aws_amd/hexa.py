@@ -474,8 +474,10 @@ class Server:
     def wait(self, job_id, want_x=False):
         r = JobResult()
         x = np.zeros(self._ne.pop(job_id)) if want_x else None
-        lib().hexa_server_wait(self._s, int(job_id), C.byref(r), _ptr(x))
+        st = lib().hexa_server_wait(self._s, int(job_id), C.byref(r), _ptr(x))
         d = r.as_dict()
+        if st != OK:
+            d["error"] = lib().hexa_last_error().decode(errors="replace")
         return (d, x) if want_x else d
 
     def close(self):
