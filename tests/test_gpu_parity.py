@@ -28,7 +28,8 @@ def rel(a, b):
                                             (hexa.KE_F32_WIDENED, hexa.MATVEC_AUTO)])
 @pytest.mark.parametrize("grid,h", [((4, 3, 2), (1.0, 1.0, 1.0)), ((7, 5, 3), (1.0, 2.0, 0.5)),
                                     ((33, 9, 5), (1.0, 1.0, 1.0)), ((15, 14, 40), (0.5, 1.0, 1.5)),
-                                    ((130, 15, 9), (1.0, 0.5, 2.0)), ((1, 1, 1), (1.0, 1.0, 1.0))])
+                                    ((130, 15, 9), (1.0, 0.5, 2.0)), ((20, 31, 6), (1.0, 1.0, 1.0)),
+                                    ((1, 1, 1), (1.0, 1.0, 1.0))])
 def test_matvec_matches_oracle(grid, h, ke_mode, kernel):
     say("matvec", grid, h, ke_mode, kernel)
     nx, ny, nz = grid
@@ -53,6 +54,7 @@ def _golden_problem(tag):
 @pytest.mark.parametrize("tag", ["a", "b"])
 @pytest.mark.parametrize("precond,precision,kernel", [(hexa.PRECOND_JACOBI, hexa.PREC_FP64, hexa.MATVEC_AUTO),
                                                       (hexa.PRECOND_MG, hexa.PREC_FP64, hexa.MATVEC_AUTO),
+                                                      (hexa.PRECOND_MG, hexa.PREC_FP64, hexa.MATVEC_DENSE),
                                                       (hexa.PRECOND_MG, hexa.PREC_MIXED, hexa.MATVEC_AUTO),
                                                       (hexa.PRECOND_MG, hexa.PREC_MIXED, hexa.MATVEC_DENSE)])
 def test_solve_matches_golden_direct(tag, precond, precision, kernel):

@@ -1,0 +1,236 @@
+// Dense-Ke elasticity operator on the matrix cores (FP64 MFMA), for element
+// matrices that are not exactly factorisable (the reference's f32-widened
+// Ke, R/src/Element/Element.jl get_canonical_stiffness in Float32 widened to
+// Float64 by the GPU solver's Ke upload).
+//
+// A block owns a 15 x 15 column of nodes and marches a chunk of node planes
+// along z.  For each element layer it takes the 16 x 16 elements around its
+// nodes as 16 groups of 16 (one element row each).  A group's element
+// outputs Y = Ke U (24 x 16, Ke padded to 32 rows) are 2 x 6
+// v_mfma_f64_16x16x4 steps: Ke is the A operand, held in VGPRs for the whole
+// launch; U, the group's 16 element DOF vectors, is gathered as the B
+// operand from the node planes staged in LDS.  Y, scaled by the element
+// factors, goes to LDS, and each owned node sums the rows of its 8 elements
+// from there: a node-centric gather, so no atomics and a fixed summation
+// order.  Node plane k is complete after element layer k; the part its 4
+// elements below contribute (layer k-1) is carried in registers.
+//
+// Per element the product costs 32 x 24 FMA on the matrix cores; at
+// 16 x 16 elements per 15 x 15 nodes that bounds the launch at about
+// 0.8 ms for 33.5 M elements (78.6 TFLOP/s FP64 matrix peak).
+#include "hexa_internal.h"
+
+namespace hexa {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int MX = 16;              // elements per group (MFMA N) and per tile row
+constexpr int MY = 16;              // groups (element rows) per tile
+constexpr int OWN = 15;             // owned nodes per tile axis
+constexpr int PN = MX + 1;          // staged nodes per plane row
+constexpr int PROW = 3 * PN;        // doubles per staged plane row
+constexpr int PLANE = PROW * (MY + 1);   // doubles per staged node plane
+constexpr int NE = MX * MY;         // elements per tile layer
+constexpr int NT = 512;             // threads: 8 waves, 2 groups each
+constexpr int NOWN = OWN * OWN;     // owned nodes per plane
+
+__device__ constexpr int MCX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__device__ constexpr int MCY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__device__ constexpr int MCZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+__device__ constexpr int corner_at(int cx, int cy, int cz) { return 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0)); }
+
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __restrict__ Ke,
+                                                     const double* __restrict__ fac,
+                                                     const uint8_t* __restrict__ mask,
+                                                     const double* __restrict__ u, const double* __restrict__ b,
+                                                     const double* __restrict__ dinv, double omega, double shift,
+                                                     double* __restrict__ y, int kchunk, int tiles_x, int tiles_y) {
+    __shared__ double planes[3 * PLANE];
+    __shared__ double Y[24 * NE];   // Y[m * NE + e], e = ey * MX + ex
+
+    const int NX = g.nx + 1, NY = g.ny + 1, NZ = g.nz + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bid = blockIdx.x;
+    const int tx = bid % tiles_x, ty = (bid / tiles_x) % tiles_y, tz = bid / (tiles_x * tiles_y);
+    const int i0 = tx * OWN - 1, j0 = ty * OWN - 1;   // global index of local node / element 0
+    const int k0 = tz * kchunk, k1 = min(k0 + kchunk, NZ);
+
+    // A operand: lane holds Ke[m][d], m = lane % 16 + 16 t, d = 4 s + lane / 16
+    const int lj = lane & 15, lk = lane >> 4;
+    double a[2][6];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const int m = lj + 16 * t;
+            a[t][s] = m < 24 ? Ke[m * 24 + 4 * s + lk] : 0.0;
+        }
+    // B operand: lane holds U[d][e] of element ex = lane % 16, d = 4 s + lane / 16,
+    // i.e. component d % 3 of corner d / 3
+    int boff[6], bhi[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int d = 4 * s + lk, c = d / 3, comp = d - 3 * c;
+        boff[s] = MCY[c] * PROW + (lj + MCX[c]) * 3 + comp;
+        bhi[s] = MCZ[c];
+    }
+
+    // staging of node plane kp into LDS slot: items tid and tid + NT
+    auto stage_load = [&](int kp, double (&v)[2]) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int it = tid + q * NT;
+            v[q] = 0.0;
+            if (it < PLANE) {
+                const int r = it / PROW, w = it - r * PROW;
+                const int nx = w / 3, comp = w - 3 * nx;
+                const int i = i0 + nx, j = j0 + r;
+                if ((unsigned)i < (unsigned)NX && (unsigned)j < (unsigned)NY && (unsigned)kp < (unsigned)NZ)
+                    v[q] = u[3 * (i + (int64_t)NX * (j + (int64_t)NY * kp)) + comp];
+            }
+        }
+    };
+    auto stage_store = [&](int slot, const double (&v)[2]) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int it = tid + q * NT;
+            if (it < PLANE) planes[slot * PLANE + it] = v[q];
+        }
+    };
+
+    // gather threads: owned node (1 + o % 15, 1 + o / 15) of the tile
+    const bool gth = tid < NOWN;
+    const int gx = 1 + tid % OWN, gy = 1 + tid / OWN;
+    const int gi = i0 + gx, gj = j0 + gy;
+    const bool gown = gth && gi < NX && gj < NY;
+    double carry[3] = {0.0, 0.0, 0.0};
+
+    {
+        double v[2];
+        stage_load(k0 - 1, v);
+        stage_store(0, v);
+        stage_load(k0, v);
+        stage_store(1, v);
+    }
+    __syncthreads();
+
+    int slot_lo = 0;   // LDS slot of node plane ek; ek + 1 and ek + 2 follow cyclically
+    for (int ek = k0 - 1; ek < k1; ++ek) {
+        const int slot_hi = slot_lo == 2 ? 0 : slot_lo + 1;
+        const int slot_nx = slot_hi == 2 ? 0 : slot_hi + 1;
+        double nxt[2];
+        stage_load(ek + 2, nxt);
+        // element factors of this wave's two groups (0 outside the grid)
+        double f[2];
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq) {
+            const int ei = i0 + lj, ej = j0 + 2 * wave + gq;
+            const bool in = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny && (unsigned)ek < (unsigned)g.nz;
+            f[gq] = in ? fac[ei + (int64_t)g.nx * (ej + (int64_t)g.ny * ek)] : 0.0;
+        }
+        // flags, right-hand side and inverse diagonal of output node plane ek
+        const bool out = gown && ek >= k0;
+        const int64_t gn = gi + (int64_t)NX * (gj + (int64_t)NY * (out ? ek : 0));
+        uint8_t mb = 0;
+        double bv[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
+        if (out) {
+            mb = mask ? mask[gn] : 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (MODE != OP_APPLY) bv[c] = b[3 * gn + c];
+                if (MODE == OP_SMOOTH) dv[c] = dinv[3 * gn + c];
+            }
+        }
+
+        // Y = Ke U for the wave's two groups
+        const int lo = slot_lo * PLANE, hi = slot_hi * PLANE;
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq) {
+            const int ey = 2 * wave + gq;
+            d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                const double bval = planes[(bhi[s] ? hi : lo) + ey * PROW + boff[s]];
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], bval, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], bval, acc1, 0, 0, 0);
+            }
+            // D[m][e]: register r of lane holds m = lane / 16 + 4 r (+16 for the
+            // second tile), e = lane % 16
+            const int e = ey * MX + lj;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Y[(lk + 4 * r) * NE + e] = f[gq] * acc0[r];
+                if (r < 2) Y[(16 + lk + 4 * r) * NE + e] = f[gq] * acc1[r];
+            }
+        }
+        stage_store(slot_nx, nxt);
+        __syncthreads();
+
+        if (gth) {
+            double bot[3] = {0.0, 0.0, 0.0}, top[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int e = (gy - 1 + dy) * MX + (gx - 1 + dx);
+                    const int cb = corner_at(1 - dx, 1 - dy, 0), ct = cb + 4;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        bot[c] += Y[(3 * cb + c) * NE + e];
+                        top[c] += Y[(3 * ct + c) * NE + e];
+                    }
+                }
+            if (out) {
+                const double* uc = &planes[lo + gy * PROW + 3 * gx];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double Au = carry[c] + bot[c] + shift * uc[c];
+                    double v;
+                    if (MODE == OP_APPLY) v = Au;
+                    else if (MODE == OP_RESID) v = bv[c] - Au;
+                    else v = uc[c] + omega * dv[c] * (bv[c] - Au);
+                    y[3 * gn + c] = ((mb >> c) & 1) ? 0.0 : v;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) carry[c] = top[c];
+        }
+        __syncthreads();
+        slot_lo = slot_hi;
+    }
+}
+
+}  // namespace
+
+bool elastic_mfma_supported(const Grid3& g) {
+    return (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) < (int64_t)1 << 31;
+}
+
+void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const double* Ke, const double* fac,
+                         const uint8_t* mask, const double* u, const double* b, const double* dinv, double omega,
+                         double shift, double* y) {
+    const int tiles_x = (g.nx + 1 + OWN - 1) / OWN, tiles_y = (g.ny + 1 + OWN - 1) / OWN;
+    const int NZ = g.nz + 1;
+    int kchunk = 32;
+    while (kchunk > 4 && (int64_t)tiles_x * tiles_y * ((NZ + kchunk - 1) / kchunk) < 2048) kchunk >>= 1;
+    const int tiles = tiles_x * tiles_y * ((NZ + kchunk - 1) / kchunk);
+    switch (mode) {
+        case OP_APPLY:
+            k_elastic_mfma<OP_APPLY><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y, kchunk,
+                                                          tiles_x, tiles_y);
+            break;
+        case OP_RESID:
+            k_elastic_mfma<OP_RESID><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y, kchunk,
+                                                          tiles_x, tiles_y);
+            break;
+        default:
+            k_elastic_mfma<OP_SMOOTH><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y, kchunk,
+                                                           tiles_x, tiles_y);
+            break;
+    }
+}
+
+}  // namespace hexa
