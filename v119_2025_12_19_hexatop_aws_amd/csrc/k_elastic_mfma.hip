@@ -10,14 +10,15 @@
 // v_mfma_f64_16x16x4 steps: Ke is the A operand, held in VGPRs for the whole
 // launch; U, the group's 16 element DOF vectors, is gathered as the B
 // operand from the node planes staged in LDS.  Y, scaled by the element
-// factors, goes to LDS, and each owned node sums the rows of its 8 elements
-// from there: a node-centric gather, so no atomics and a fixed summation
-// order.  Node plane k is complete after element layer k; the part its 4
-// elements below contribute (layer k-1) is carried in registers.
+// factors, goes to LDS, and each owned node's DOF sums its rows of the 8
+// elements around the node from there (all threads share the 15 x 15 x 3
+// DOFs): a node-centric gather, so no atomics and a fixed summation order.
+// Node plane k is complete after element layer k; the part its 4 elements
+// below contribute (layer k-1) is carried in registers.
 //
 // Per element the product costs 32 x 24 FMA on the matrix cores; at
 // 16 x 16 elements per 15 x 15 nodes that bounds the launch at about
-// 0.8 ms for 33.5 M elements (78.6 TFLOP/s FP64 matrix peak).
+// 0.84 ms for 33.5 M elements (78.6 TFLOP/s FP64 matrix peak).
 #include "hexa_internal.h"
 
 namespace hexa {
@@ -35,6 +36,8 @@ constexpr int PLANE = PROW * (MY + 1);   // doubles per staged node plane
 constexpr int NE = MX * MY;         // elements per tile layer
 constexpr int NT = 512;             // threads: 8 waves, 2 groups each
 constexpr int NOWN = OWN * OWN;     // owned nodes per plane
+constexpr int NES = NE + 16;        // Y row stride in doubles, padded so the 4 rows a
+                                    // write instruction spans fall in different LDS banks
 
 __device__ constexpr int MCX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 __device__ constexpr int MCY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
@@ -49,7 +52,7 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
                                                      const double* __restrict__ dinv, double omega, double shift,
                                                      double* __restrict__ y, int kchunk, int tiles_x, int tiles_y) {
     __shared__ double planes[3 * PLANE];
-    __shared__ double Y[24 * NE];   // Y[m * NE + e], e = ey * MX + ex
+    __shared__ double Y[24 * NES];   // Y[m * NES + e], e = ey * MX + ex
 
     const int NX = g.nx + 1, NY = g.ny + 1, NZ = g.nz + 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -101,12 +104,21 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
         }
     };
 
-    // gather threads: owned node (1 + o % 15, 1 + o / 15) of the tile
-    const bool gth = tid < NOWN;
-    const int gx = 1 + tid % OWN, gy = 1 + tid / OWN;
-    const int gi = i0 + gx, gj = j0 + gy;
-    const bool gown = gth && gi < NX && gj < NY;
-    double carry[3] = {0.0, 0.0, 0.0};
+    // gather items: (owned node o, component c), item = 3 o + c, items tid and tid + NT
+    int gq_n[2], gq_c[2];
+    bool gq_in[2];
+    int64_t gq_base[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int itm = tid + q * NT;
+        const int o = itm / 3;
+        gq_c[q] = itm - 3 * o;
+        gq_n[q] = o;
+        const int gx = 1 + o % OWN, gy = 1 + o / OWN;
+        gq_in[q] = itm < 3 * NOWN && i0 + gx < NX && j0 + gy < NY;
+        gq_base[q] = (i0 + gx) + (int64_t)NX * (j0 + gy);
+    }
+    double carry[2] = {0.0, 0.0};
 
     {
         double v[2];
@@ -131,72 +143,74 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
             const bool in = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny && (unsigned)ek < (unsigned)g.nz;
             f[gq] = in ? fac[ei + (int64_t)g.nx * (ej + (int64_t)g.ny * ek)] : 0.0;
         }
-        // flags, right-hand side and inverse diagonal of output node plane ek
-        const bool out = gown && ek >= k0;
-        const int64_t gn = gi + (int64_t)NX * (gj + (int64_t)NY * (out ? ek : 0));
-        uint8_t mb = 0;
-        double bv[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
-        if (out) {
-            mb = mask ? mask[gn] : 0;
+        // flag, right-hand side and inverse diagonal of the items of output plane ek
+        uint8_t mb[2] = {0, 0};
+        double bv[2] = {0.0, 0.0}, dv[2] = {0.0, 0.0};
+        int64_t gdof[2];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                if (MODE != OP_APPLY) bv[c] = b[3 * gn + c];
-                if (MODE == OP_SMOOTH) dv[c] = dinv[3 * gn + c];
+        for (int q = 0; q < 2; ++q) {
+            const bool o = gq_in[q] && ek >= k0;
+            const int64_t gn = gq_base[q] + (int64_t)NX * NY * (o ? ek : 0);
+            gdof[q] = 3 * gn + gq_c[q];
+            if (o) {
+                mb[q] = mask ? (mask[gn] >> gq_c[q]) & 1 : 0;
+                if (MODE != OP_APPLY) bv[q] = b[gdof[q]];
+                if (MODE == OP_SMOOTH) dv[q] = dinv[gdof[q]];
             }
         }
 
         // Y = Ke U for the wave's two groups
         const int lo = slot_lo * PLANE, hi = slot_hi * PLANE;
+        d4 acc[2][2];
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq) acc[gq][0] = acc[gq][1] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const int base = (bhi[s] ? hi : lo) + 2 * wave * PROW + boff[s];
+            const double b0 = planes[base], b1 = planes[base + PROW];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b0, acc[0][0], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b1, acc[1][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b0, acc[0][1], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b1, acc[1][1], 0, 0, 0);
+        }
 #pragma unroll
         for (int gq = 0; gq < 2; ++gq) {
-            const int ey = 2 * wave + gq;
-            d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int s = 0; s < 6; ++s) {
-                const double bval = planes[(bhi[s] ? hi : lo) + ey * PROW + boff[s]];
-                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], bval, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], bval, acc1, 0, 0, 0);
-            }
-            // D[m][e]: register r of lane holds m = lane / 16 + 4 r (+16 for the
-            // second tile), e = lane % 16
-            const int e = ey * MX + lj;
+            const int e = (2 * wave + gq) * MX + lj;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                Y[(lk + 4 * r) * NE + e] = f[gq] * acc0[r];
-                if (r < 2) Y[(16 + lk + 4 * r) * NE + e] = f[gq] * acc1[r];
+                Y[(lk + 4 * r) * NES + e] = f[gq] * acc[gq][0][r];
+                if (r < 2) Y[(16 + lk + 4 * r) * NES + e] = f[gq] * acc[gq][1][r];
             }
         }
         stage_store(slot_nx, nxt);
         __syncthreads();
 
-        if (gth) {
-            double bot[3] = {0.0, 0.0, 0.0}, top[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int dy = 0; dy < 2; ++dy)
+        for (int q = 0; q < 2; ++q) {
+            if (tid + q * NT < 3 * NOWN) {
+                const int o = gq_n[q], c = gq_c[q];
+                const int gx = 1 + o % OWN, gy = 1 + o / OWN;
+                double bot = 0.0, top = 0.0;
 #pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                    const int e = (gy - 1 + dy) * MX + (gx - 1 + dx);
-                    const int cb = corner_at(1 - dx, 1 - dy, 0), ct = cb + 4;
+                for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        bot[c] += Y[(3 * cb + c) * NE + e];
-                        top[c] += Y[(3 * ct + c) * NE + e];
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int e = (gy - 1 + dy) * MX + (gx - 1 + dx);
+                        const int cb = corner_at(1 - dx, 1 - dy, 0), ct = cb + 4;
+                        bot += Y[(3 * cb + c) * NES + e];
+                        top += Y[(3 * ct + c) * NES + e];
                     }
-                }
-            if (out) {
-                const double* uc = &planes[lo + gy * PROW + 3 * gx];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double Au = carry[c] + bot[c] + shift * uc[c];
+                if (gq_in[q] && ek >= k0) {
+                    const double uc = planes[lo + gy * PROW + 3 * gx + c];
+                    const double Au = carry[q] + bot + shift * uc;
                     double v;
                     if (MODE == OP_APPLY) v = Au;
-                    else if (MODE == OP_RESID) v = bv[c] - Au;
-                    else v = uc[c] + omega * dv[c] * (bv[c] - Au);
-                    y[3 * gn + c] = ((mb >> c) & 1) ? 0.0 : v;
+                    else if (MODE == OP_RESID) v = bv[q] - Au;
+                    else v = uc + omega * dv[q] * (bv[q] - Au);
+                    y[gdof[q]] = mb[q] ? 0.0 : v;
                 }
+                carry[q] = top;
             }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) carry[c] = top[c];
         }
         __syncthreads();
         slot_lo = slot_hi;
