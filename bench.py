@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "matvec HBM GB/s; time-to-solve @1e-8 rel res, 33.5M hex; served solves/s + p99"
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
 
 
 def log(*a):
@@ -208,6 +209,17 @@ def main():
     achieved = nbytes64 / (ms_kernel * 1e-3) / 1e9
     log(f"FP64 matvec: {ms_kernel:.3f} ms/launch (events), {ms_step:.3f} ms/step wall, {achieved:.0f} GB/s; "
         f"FP32 matvec {ms32:.3f} ms ({nbytes32 / ms32 / 1e6:.0f} GB/s)")
+    # the reference-shaped entry point's default element matrix (Float32 Ke
+    # widened to FP64) is not factorisable: the dense product on the matrix
+    # cores, 1152 useful flop per element
+    with hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu, ke_mode=hexa.KE_F32_WIDENED) as sd:
+        sd.set_density(p.rho)
+        sd.matvec_dev(x.data_ptr(), y.data_ptr(), hexa.PREC_FP64, reps=max(1, args.warmup))
+        ms_dense = sd.matvec_dev(x.data_ptr(), y.data_ptr(), hexa.PREC_FP64, reps=args.steps)
+    dense = {"kernel": "k_elastic_mfma<APPLY>", "kernel_ms": ms_dense,
+             "useful_TFLOPs": 1152.0 * p.n_elem / (ms_dense * 1e-3) / 1e12, "bound": "mfma",
+             "peak_TFLOPs": FP64_MFMA_PEAK_TFLOPS}
+    log(f"dense-Ke (f32-widened) FP64 matvec: {ms_dense:.3f} ms, {dense['useful_TFLOPs']:.1f} TFLOP/s useful")
     if args.traffic_probe:
         y.copy_(x)
         torch.cuda.synchronize()
@@ -289,6 +301,7 @@ def main():
                      "kernel": "k_elastic_rw<double>", "bytes_per_launch": nbytes64,
                      "kernel_ms": ms_kernel},
         "matvec_fp32": {"kernel_ms": ms32, "GBps": nbytes32 / (ms32 * 1e-3) / 1e9},
+        "matvec_dense_widened_ke": dense,
         "solve": solves,
         "shard": shard,
         "load_cases": cases,
