@@ -62,16 +62,20 @@ GRIDS = [((33, 9, 5), (1.0, 1.0, 1.0)), ((130, 15, 9), (1.0, 0.5, 2.0)), ((64, 2
          ((63, 8, 40), (2.0, 1.0, 1.0))]
 
 
+@pytest.mark.parametrize("kernel", [hexa.MATVEC_AUTO, hexa.MATVEC_DENSE])
 @pytest.mark.parametrize("grid,h", GRIDS)
-def test_fp32_matvec_matches_oracle(grid, h):
-    say("fp32 matvec", grid, h)
+def test_fp32_matvec_matches_oracle(grid, h, kernel):
+    """FP32 product: factored row-wave kernel (AUTO) and the dense-Ke
+    matrix-core kernel."""
+    say("fp32 matvec", grid, h, kernel)
     hip = _hip()
     nx, ny, nz = grid
     rng = np.random.default_rng(21)
     rho = rng.uniform(0.001, 1.0, nx * ny * nz)
     nd = 3 * (nx + 1) * (ny + 1) * (nz + 1)
     x = rng.standard_normal(nd).astype(np.float32)
-    with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, precision=hexa.PREC_MIXED) as s:
+    with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, precision=hexa.PREC_MIXED,
+                     matvec_kernel=kernel) as s:
         s.set_density(rho)
         dx, dy = DevBuf(hip, x), DevBuf(hip, np.zeros(nd, np.float32))
         s.matvec_dev(dx.p.value, dy.p.value, hexa.PREC_MIXED, 2)
@@ -108,19 +112,22 @@ def test_mixed_vcycle_matches_fp64_vcycle(grid, h):
     assert np.all(np.isfinite(a)) and err <= 5e-3
 
 
+@pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
 @pytest.mark.parametrize("grid,h", GRIDS[1:])
-def test_dense_vcycle_matches_factored_vcycle(grid, h):
-    """The FP64 V-cycle built on the dense-Ke (matrix-core) operator equals
-    the one on the factored operator: same Ke, same smoothing and residual
-    steps, summed in another order."""
-    say("dense v-cycle", grid, h)
+def test_dense_vcycle_matches_factored_vcycle(grid, h, precision):
+    """The V-cycle built on the dense-Ke (matrix-core) operator equals the one
+    on the factored operator: same Ke, same smoothing and residual steps,
+    summed in another order (FP64: to rounding; mixed: FP32 sweeps whose
+    rounding differences the V-cycle carries, still far below an O(1)
+    mistake)."""
+    say("dense v-cycle", grid, h, precision)
     nx, ny, nz = grid
     p = P.cantilever(nx, ny, nz, seed=9, smooth_passes=2)
     rng = np.random.default_rng(6)
     r = rng.standard_normal(p.n_dof) * (1 - p.fixed)
     z = {}
     for kern in (hexa.MATVEC_FACTORED, hexa.MATVEC_DENSE):
-        with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, matvec_kernel=kern) as s:
+        with hexa.Solver(nx, ny, nz, h, E0=1.0, Emin=1e-9, penal=3.0, matvec_kernel=kern, precision=precision) as s:
             s.set_problem(p.fixed, p.f[0])
             s.set_density(p.rho)
             s.solve(tol=1e-8, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2)
@@ -128,4 +135,4 @@ def test_dense_vcycle_matches_factored_vcycle(grid, h):
     a, b = z[hexa.MATVEC_DENSE], z[hexa.MATVEC_FACTORED]
     err = np.abs(a - b).max() / np.abs(b).max()
     say("max rel diff", err)
-    assert err <= 1e-11
+    assert np.all(np.isfinite(a)) and err <= (1e-11 if precision == hexa.PREC_FP64 else 1e-4)

@@ -117,16 +117,16 @@ struct PcgState {
 
 // ---- kernel launchers (implemented in the .hip files) -----------------
 // Elasticity (k_elastic.hip)
-// (FP64 with an FP64 diagonal runs on the matrix cores, k_elastic_mfma.hip;
-// OP_SMOOTH0 is not provided by the dense kernels)
+// (runs on the matrix cores, k_elastic_mfma.hip; OP_SMOOTH0 is not provided
+// by the dense kernels)
 template <typename T, typename DT>
 void launch_elastic_op(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, const T* fac,
                        const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega,
                        T shift, T* y);
 bool elastic_mfma_supported(const Grid3& g);
-void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const double* Ke, const double* fac,
-                         const uint8_t* mask, const double* u, const double* b, const double* dinv, double omega,
-                         double shift, double* y);
+template <typename T, typename DT>
+void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, const T* fac, const uint8_t* mask,
+                         const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
 template <typename T, typename DT>
 void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                          const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);

@@ -11,8 +11,6 @@
 // Ke is read through uniform (scalar-cache) loads: every lane of a wave reads
 // the same Ke entry at the same time, so the entries live in SGPRs and each
 // v_fma takes one scalar operand; LDS bandwidth is left to the data.
-#include <type_traits>
-
 #include "hexa_internal.h"
 
 namespace hexa {
@@ -278,11 +276,9 @@ template <typename T, typename DT>
 void launch_elastic_op(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, const T* fac,
                        const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega,
                        T shift, T* y) {
-    if constexpr (std::is_same<T, double>::value && std::is_same<DT, double>::value) {
-        if (elastic_mfma_supported(g)) {
-            launch_elastic_mfma(s, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
-            return;
-        }
+    if (elastic_mfma_supported(g)) {
+        launch_elastic_mfma<T, DT>(s, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
+        return;
     }
     dim3 block(64, 4, 1);
     dim3 grid((unsigned)((g.nx + 1 + 63) / 64), (unsigned)((g.ny + 1 + 3) / 4), (unsigned)(g.nz + 1));

@@ -1,6 +1,7 @@
-// Dense-Ke elasticity operator on the matrix cores (FP64 MFMA), for element
-// matrices that are not exactly factorisable (the reference's f32-widened
-// Ke, R/src/Element/Element.jl get_canonical_stiffness in Float32 widened to
+// Dense-Ke elasticity operator on the matrix cores (FP64 MFMA, and FP32 MFMA
+// for the FP32 levels of a mixed V-cycle), for element matrices that are not
+// exactly factorisable (the reference's f32-widened Ke,
+// R/src/Element/Element.jl get_canonical_stiffness in Float32 widened to
 // Float64 by the GPU solver's Ke upload).
 //
 // A block owns a 15 x 15 column of nodes and marches a chunk of node planes
@@ -26,17 +27,42 @@ namespace hexa {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core step per precision, and the row of the 16 x 16
+// result that accumulator register r of lane group lk = lane / 16 holds
+// (column lane % 16 in both).  The two precisions differ: checked on gfx950
+// with a one-wave probe of each builtin.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef d4 acc;
+    __device__ static acc step(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static constexpr int row(int lk, int r) { return lk + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    typedef f4 acc;
+    __device__ static acc step(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static constexpr int row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <typename T, typename DT>
+__device__ __forceinline__ T to_value(DT v) { return (T)v; }
+template <>
+__device__ __forceinline__ float to_value<float, uint16_t>(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
 
 constexpr int MX = 16;              // elements per group (MFMA N) and per tile row
 constexpr int MY = 16;              // groups (element rows) per tile
 constexpr int OWN = 15;             // owned nodes per tile axis
 constexpr int PN = MX + 1;          // staged nodes per plane row
-constexpr int PROW = 3 * PN;        // doubles per staged plane row
-constexpr int PLANE = PROW * (MY + 1);   // doubles per staged node plane
+constexpr int PROW = 3 * PN;        // values per staged plane row
+constexpr int PLANE = PROW * (MY + 1);   // values per staged node plane
 constexpr int NE = MX * MY;         // elements per tile layer
 constexpr int NT = 512;             // threads: 8 waves, 2 groups each
 constexpr int NOWN = OWN * OWN;     // owned nodes per plane
-constexpr int NES = NE + 16;        // Y row stride in doubles, padded so the 4 rows a
+constexpr int NES = NE + 16;        // Y row stride in values, padded so the 4 rows a
                                     // write instruction spans fall in different LDS banks
 
 __device__ constexpr int MCX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
@@ -44,15 +70,15 @@ __device__ constexpr int MCY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __device__ constexpr int MCZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 __device__ constexpr int corner_at(int cx, int cy, int cz) { return 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0)); }
 
-template <int MODE>
-__global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __restrict__ Ke,
-                                                     const double* __restrict__ fac,
-                                                     const uint8_t* __restrict__ mask,
-                                                     const double* __restrict__ u, const double* __restrict__ b,
-                                                     const double* __restrict__ dinv, double omega, double shift,
-                                                     double* __restrict__ y, int kchunk, int tiles_x, int tiles_y) {
-    __shared__ double planes[3 * PLANE];
-    __shared__ double Y[24 * NES];   // Y[m * NES + e], e = ey * MX + ex
+template <typename T, typename DT, int MODE>
+__global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const T* __restrict__ Ke, const T* __restrict__ fac,
+                                                     const uint8_t* __restrict__ mask, const T* __restrict__ u,
+                                                     const T* __restrict__ b, const DT* __restrict__ dinv, T omega,
+                                                     T shift, T* __restrict__ y, int kchunk, int tiles_x,
+                                                     int tiles_y) {
+    typedef Mfma<T> M;
+    __shared__ T planes[3 * PLANE];
+    __shared__ T Y[24 * NES];   // Y[m * NES + e], e = ey * MX + ex
 
     const int NX = g.nx + 1, NY = g.ny + 1, NZ = g.nz + 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -63,13 +89,13 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
 
     // A operand: lane holds Ke[m][d], m = lane % 16 + 16 t, d = 4 s + lane / 16
     const int lj = lane & 15, lk = lane >> 4;
-    double a[2][6];
+    T a[2][6];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             const int m = lj + 16 * t;
-            a[t][s] = m < 24 ? Ke[m * 24 + 4 * s + lk] : 0.0;
+            a[t][s] = m < 24 ? Ke[m * 24 + 4 * s + lk] : T(0);
         }
     // B operand: lane holds U[d][e] of element ex = lane % 16, d = 4 s + lane / 16,
     // i.e. component d % 3 of corner d / 3
@@ -82,11 +108,11 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
     }
 
     // staging of node plane kp into LDS slot: items tid and tid + NT
-    auto stage_load = [&](int kp, double (&v)[2]) {
+    auto stage_load = [&](int kp, T (&v)[2]) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int it = tid + q * NT;
-            v[q] = 0.0;
+            v[q] = T(0);
             if (it < PLANE) {
                 const int r = it / PROW, w = it - r * PROW;
                 const int nx = w / 3, comp = w - 3 * nx;
@@ -96,7 +122,7 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
             }
         }
     };
-    auto stage_store = [&](int slot, const double (&v)[2]) {
+    auto stage_store = [&](int slot, const T (&v)[2]) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int it = tid + q * NT;
@@ -118,10 +144,10 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
         gq_in[q] = itm < 3 * NOWN && i0 + gx < NX && j0 + gy < NY;
         gq_base[q] = (i0 + gx) + (int64_t)NX * (j0 + gy);
     }
-    double carry[2] = {0.0, 0.0};
+    T carry[2] = {T(0), T(0)};
 
     {
-        double v[2];
+        T v[2];
         stage_load(k0 - 1, v);
         stage_store(0, v);
         stage_load(k0, v);
@@ -133,19 +159,19 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
     for (int ek = k0 - 1; ek < k1; ++ek) {
         const int slot_hi = slot_lo == 2 ? 0 : slot_lo + 1;
         const int slot_nx = slot_hi == 2 ? 0 : slot_hi + 1;
-        double nxt[2];
+        T nxt[2];
         stage_load(ek + 2, nxt);
         // element factors of this wave's two groups (0 outside the grid)
-        double f[2];
+        T f[2];
 #pragma unroll
         for (int gq = 0; gq < 2; ++gq) {
             const int ei = i0 + lj, ej = j0 + 2 * wave + gq;
             const bool in = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny && (unsigned)ek < (unsigned)g.nz;
-            f[gq] = in ? fac[ei + (int64_t)g.nx * (ej + (int64_t)g.ny * ek)] : 0.0;
+            f[gq] = in ? fac[ei + (int64_t)g.nx * (ej + (int64_t)g.ny * ek)] : T(0);
         }
         // flag, right-hand side and inverse diagonal of the items of output plane ek
         uint8_t mb[2] = {0, 0};
-        double bv[2] = {0.0, 0.0}, dv[2] = {0.0, 0.0};
+        T bv[2] = {T(0), T(0)}, dv[2] = {T(0), T(0)};
         int64_t gdof[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -155,31 +181,32 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
             if (o) {
                 mb[q] = mask ? (mask[gn] >> gq_c[q]) & 1 : 0;
                 if (MODE != OP_APPLY) bv[q] = b[gdof[q]];
-                if (MODE == OP_SMOOTH) dv[q] = dinv[gdof[q]];
+                if (MODE == OP_SMOOTH) dv[q] = to_value<T, DT>(dinv[gdof[q]]);
             }
         }
 
         // Y = Ke U for the wave's two groups
         const int lo = slot_lo * PLANE, hi = slot_hi * PLANE;
-        d4 acc[2][2];
+        typename M::acc acc[2][2];
 #pragma unroll
-        for (int gq = 0; gq < 2; ++gq) acc[gq][0] = acc[gq][1] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int gq = 0; gq < 2; ++gq) acc[gq][0] = acc[gq][1] = typename M::acc{T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             const int base = (bhi[s] ? hi : lo) + 2 * wave * PROW + boff[s];
-            const double b0 = planes[base], b1 = planes[base + PROW];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b0, acc[0][0], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0][s], b1, acc[1][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b0, acc[0][1], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1][s], b1, acc[1][1], 0, 0, 0);
+            const T b0 = planes[base], b1 = planes[base + PROW];
+            acc[0][0] = M::step(a[0][s], b0, acc[0][0]);
+            acc[1][0] = M::step(a[0][s], b1, acc[1][0]);
+            acc[0][1] = M::step(a[1][s], b0, acc[0][1]);
+            acc[1][1] = M::step(a[1][s], b1, acc[1][1]);
         }
 #pragma unroll
         for (int gq = 0; gq < 2; ++gq) {
             const int e = (2 * wave + gq) * MX + lj;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                Y[(lk + 4 * r) * NES + e] = f[gq] * acc[gq][0][r];
-                if (r < 2) Y[(16 + lk + 4 * r) * NES + e] = f[gq] * acc[gq][1][r];
+                const int m = M::row(lk, r);
+                Y[m * NES + e] = f[gq] * acc[gq][0][r];
+                if (m < 8) Y[(16 + m) * NES + e] = f[gq] * acc[gq][1][r];
             }
         }
         stage_store(slot_nx, nxt);
@@ -190,7 +217,7 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
             if (tid + q * NT < 3 * NOWN) {
                 const int o = gq_n[q], c = gq_c[q];
                 const int gx = 1 + o % OWN, gy = 1 + o / OWN;
-                double bot = 0.0, top = 0.0;
+                T bot = T(0), top = T(0);
 #pragma unroll
                 for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
@@ -201,13 +228,13 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const double* __re
                         top += Y[(3 * ct + c) * NES + e];
                     }
                 if (gq_in[q] && ek >= k0) {
-                    const double uc = planes[lo + gy * PROW + 3 * gx + c];
-                    const double Au = carry[q] + bot + shift * uc;
-                    double v;
+                    const T uc = planes[lo + gy * PROW + 3 * gx + c];
+                    const T Au = carry[q] + bot + shift * uc;
+                    T v;
                     if (MODE == OP_APPLY) v = Au;
                     else if (MODE == OP_RESID) v = bv[q] - Au;
                     else v = uc + omega * dv[q] * (bv[q] - Au);
-                    y[gdof[q]] = mb[q] ? 0.0 : v;
+                    y[gdof[q]] = mb[q] ? T(0) : v;
                 }
                 carry[q] = top;
             }
@@ -223,9 +250,9 @@ bool elastic_mfma_supported(const Grid3& g) {
     return (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) < (int64_t)1 << 31;
 }
 
-void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const double* Ke, const double* fac,
-                         const uint8_t* mask, const double* u, const double* b, const double* dinv, double omega,
-                         double shift, double* y) {
+template <typename T, typename DT>
+void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke, const T* fac, const uint8_t* mask,
+                         const T* u, const T* b, const DT* dinv, T omega, T shift, T* y) {
     const int tiles_x = (g.nx + 1 + OWN - 1) / OWN, tiles_y = (g.ny + 1 + OWN - 1) / OWN;
     const int NZ = g.nz + 1;
     int kchunk = 32;
@@ -233,18 +260,25 @@ void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const doubl
     const int tiles = tiles_x * tiles_y * ((NZ + kchunk - 1) / kchunk);
     switch (mode) {
         case OP_APPLY:
-            k_elastic_mfma<OP_APPLY><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y, kchunk,
-                                                          tiles_x, tiles_y);
+            k_elastic_mfma<T, DT, OP_APPLY><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y,
+                                                                 kchunk, tiles_x, tiles_y);
             break;
         case OP_RESID:
-            k_elastic_mfma<OP_RESID><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y, kchunk,
-                                                          tiles_x, tiles_y);
+            k_elastic_mfma<T, DT, OP_RESID><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y,
+                                                                 kchunk, tiles_x, tiles_y);
             break;
         default:
-            k_elastic_mfma<OP_SMOOTH><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y, kchunk,
-                                                           tiles_x, tiles_y);
+            k_elastic_mfma<T, DT, OP_SMOOTH><<<tiles, NT, 0, s>>>(g, Ke, fac, mask, u, b, dinv, omega, shift, y,
+                                                                  kchunk, tiles_x, tiles_y);
             break;
     }
 }
+
+#define HX_INST_MFMA(T, DT)                                                                                     \
+    template void launch_elastic_mfma<T, DT>(hipStream_t, const Grid3&, OpMode, const T*, const T*, const uint8_t*, \
+                                             const T*, const T*, const DT*, T, T, T*);
+HX_INST_MFMA(double, double)
+HX_INST_MFMA(float, float)
+HX_INST_MFMA(float, uint16_t)
 
 }  // namespace hexa
