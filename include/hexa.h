@@ -248,6 +248,10 @@ typedef struct {
  * (nE, host, in/out).  The filter is linear and symmetric, so F^T = F and
  * the volume of the filtered design is sum (F 1)_e x_e. */
 int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc_report* rep);
+/* The same step with the design in device memory (d_x, nE doubles, in/out):
+ * no host transfer of x.  No counterpart in the reference; for callers that
+ * keep the design on the GPU across iterations (the job server). */
+int32_t hexa_oc_step_dev(hexa_handle* h, double* d_x, const hexa_oc_params* p, hexa_oc_report* rep);
 
 /* ---- reference stress-ratio density update -------------------------------
  * Replaces TopologyOptimization.update_density! (TopOpt.jl:211-482) on the

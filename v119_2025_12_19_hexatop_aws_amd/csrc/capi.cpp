@@ -859,7 +859,21 @@ int32_t hexa_destroy(hexa_handle* h) {
 int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double* f,
                          const uint8_t* active_elem) {
     if (!h || !f) { set_error("null argument"); return HEXA_E_INVALID; }
+    return set_problem_impl(h, fixed_dof, f, 0, nullptr, nullptr, active_elem);
+}
+
+}  // extern "C"
+
+namespace hexa {
+
+// hexa_set_problem with the load given either densely (f, nd x nrhs) or as
+// nsp entries f[sidx[i]] = sval[i] of the first right-hand side, the rest 0
+// (uploads a few values instead of the whole vector).
+int32_t set_problem_impl(hexa_handle* h, const uint8_t* fixed_dof, const double* f, int64_t nsp,
+                         const int64_t* sidx, const double* sval, const uint8_t* active_elem) {
     const int64_t nd = 3 * h->g.nn;
+    for (int64_t i = 0; i < nsp; ++i)
+        if (sidx[i] < 0 || sidx[i] >= nd) { set_error("load index out of range"); return HEXA_E_INVALID; }
     HX_TRY(dalloc(h, &h->d_mask, (size_t)h->g.nn));
     HX_TRY(dalloc(h, &h->d_f, (size_t)(nd * h->opt.nrhs)));
     HX_TRY(dalloc(h, &h->d_fixed, (size_t)nd));
@@ -874,7 +888,13 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
     }
     const uint8_t* act = active_elem ? h->d_active : nullptr;
     launch_node_mask(st, h->g, h->d_fixed, act, h->d_mask);
-    HX_CHECK(hipMemcpyAsync(h->d_f, f, sizeof(double) * nd * h->opt.nrhs, hipMemcpyHostToDevice, st));
+    if (f) {
+        HX_CHECK(hipMemcpyAsync(h->d_f, f, sizeof(double) * nd * h->opt.nrhs, hipMemcpyHostToDevice, st));
+    } else {
+        launch_fill<double>(st, nd * h->opt.nrhs, 0.0, h->d_f);
+        for (int64_t i = 0; i < nsp; ++i)
+            HX_CHECK(hipMemcpyAsync(h->d_f + sidx[i], sval + i, sizeof(double), hipMemcpyHostToDevice, st));
+    }
     HX_TRY(check_launch());
     HX_CHECK(hipStreamSynchronize(st));
     h->have_problem = true;
@@ -885,6 +905,9 @@ int32_t hexa_set_problem(hexa_handle* h, const uint8_t* fixed_dof, const double*
     return HEXA_OK;
 }
 
+}  // namespace hexa
+
+extern "C" {
 
 int32_t hexa_prune_voids(hexa_handle* h, double min_stiffness_threshold, int64_t* n_active) {
     if (!h) { set_error("null handle"); return HEXA_E_INVALID; }

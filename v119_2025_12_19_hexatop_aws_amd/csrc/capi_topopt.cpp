@@ -274,21 +274,28 @@ int32_t hexa_oc_update(hexa_handle* h, double* x, const double* dc, const double
     return HEXA_OK;
 }
 
-int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc_report* rep) {
-    if (!h || !x || !p) { set_error("null argument"); return HEXA_E_INVALID; }
+}  // extern "C"
+
+namespace hexa {
+
+int32_t oc_check(hexa_handle* h, const hexa_oc_params* p) {
     if (!h->have_problem) { set_error("hexa_set_problem has not been called"); return HEXA_E_INVALID; }
     if (!(p->volfrac > 0 && p->volfrac <= 1) || !(p->move > 0) || !(p->rmin >= 0) || !(p->filter_tol > 0) ||
         !(p->bisect_tol > 0) || p->filter_max_iter < 1) {
         set_error("invalid OC step parameters");
         return HEXA_E_INVALID;
     }
-    const double t0 = now_ms();
     HX_TRY(ensure_elem(h));
     HX_TRY(dalloc(h, &h->d_rho, (size_t)h->g.ne));
+    return HEXA_OK;
+}
+
+// One SIMP/OC iteration on the design held in h->d_e0 (in/out); returns with
+// the stream synchronized and rep filled.
+int32_t oc_step_core(hexa_handle* h, const hexa_oc_params* p, hexa_oc_report* rep, double t0) {
     hipStream_t st = h->stream;
     const int64_t ne = h->g.ne;
     hexa_oc_report R{};
-    HX_CHECK(hipMemcpyAsync(h->d_e0, x, sizeof(double) * ne, hipMemcpyHostToDevice, st));
     // physical density x~ = F(x)
     int fit = 0;
     double frel = 0.0;
@@ -317,7 +324,6 @@ int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc
     HX_TRY(check_launch());
     double out[5];
     HX_CHECK(hipMemcpyAsync(out, h->d_scal, sizeof out, hipMemcpyDeviceToHost, st));
-    HX_CHECK(hipMemcpyAsync(x, h->d_e0, sizeof(double) * ne, hipMemcpyDeviceToHost, st));
     HX_CHECK(hipStreamSynchronize(st));
     R.change = out[0];
     R.volume = out[1] / (double)ne;
@@ -325,6 +331,36 @@ int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc
     R.compliance = out[4];
     R.step_ms = now_ms() - t0;
     if (rep) *rep = R;
+    return ss;
+}
+
+}  // namespace hexa
+
+extern "C" {
+
+int32_t hexa_oc_step(hexa_handle* h, double* x, const hexa_oc_params* p, hexa_oc_report* rep) {
+    if (!h || !x || !p) { set_error("null argument"); return HEXA_E_INVALID; }
+    const double t0 = now_ms();
+    HX_TRY(oc_check(h, p));
+    const size_t bytes = sizeof(double) * h->g.ne;
+    HX_CHECK(hipMemcpyAsync(h->d_e0, x, bytes, hipMemcpyHostToDevice, h->stream));
+    const int32_t ss = oc_step_core(h, p, rep, t0);
+    if (ss != HEXA_OK && ss != HEXA_E_NOT_CONVERGED) return ss;
+    HX_CHECK(hipMemcpyAsync(x, h->d_e0, bytes, hipMemcpyDeviceToHost, h->stream));
+    HX_CHECK(hipStreamSynchronize(h->stream));
+    return ss;
+}
+
+int32_t hexa_oc_step_dev(hexa_handle* h, double* d_x, const hexa_oc_params* p, hexa_oc_report* rep) {
+    if (!h || !d_x || !p) { set_error("null argument"); return HEXA_E_INVALID; }
+    const double t0 = now_ms();
+    HX_TRY(oc_check(h, p));
+    const size_t bytes = sizeof(double) * h->g.ne;
+    HX_CHECK(hipMemcpyAsync(h->d_e0, d_x, bytes, hipMemcpyDeviceToDevice, h->stream));
+    const int32_t ss = oc_step_core(h, p, rep, t0);
+    if (ss != HEXA_OK && ss != HEXA_E_NOT_CONVERGED) return ss;
+    HX_CHECK(hipMemcpyAsync(d_x, h->d_e0, bytes, hipMemcpyDeviceToDevice, h->stream));
+    HX_CHECK(hipStreamSynchronize(h->stream));
     return ss;
 }
 

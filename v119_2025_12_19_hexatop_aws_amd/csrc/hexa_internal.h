@@ -256,4 +256,11 @@ void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, doubl
                              double* x = nullptr);
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best);
 
+// Library-internal entry points shared by the C API files and the job server
+// (capi.cpp, capi_topopt.cpp).
+int32_t set_problem_impl(hexa_handle* h, const uint8_t* fixed_dof, const double* f, int64_t nsp,
+                         const int64_t* sidx, const double* sval, const uint8_t* active_elem);
+int32_t oc_check(hexa_handle* h, const hexa_oc_params* p);
+int32_t oc_step_core(hexa_handle* h, const hexa_oc_params* p, hexa_oc_report* rep, double t0);
+
 }  // namespace hexa

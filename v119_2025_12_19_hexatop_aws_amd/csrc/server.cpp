@@ -81,32 +81,35 @@ namespace {
 
 double wall_ms(const hexa_server* s) { return now_ms() - s->t0; }
 
-// Builds the job's arrays: support on one face, 1-4 unit point loads at
-// seeded nodes of the opposite face with seeded directions.
-void make_problem(const hexa_job& d, std::vector<uint8_t>& fixed, std::vector<double>& f) {
+// Builds the job's problem: support on one face (fixed, 3 nn flags), 1-4
+// unit point loads at seeded nodes of the opposite face with seeded
+// directions, as (DOF, value) pairs of the load vector.
+void make_problem(const hexa_job& d, std::vector<uint8_t>& fixed, std::vector<int64_t>& fidx,
+                  std::vector<double>& fval) {
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const int64_t NX = nx + 1, NY = ny + 1, NZ = nz + 1, nn = NX * NY * NZ;
     fixed.assign(3 * nn, 0);
-    f.assign(3 * nn, 0.0);
     const int face = ((d.support_face % 6) + 6) % 6;
     const int axis = face / 2;
     const bool hi = face % 2;
-    for (int64_t k = 0; k < NZ; ++k)
-        for (int64_t j = 0; j < NY; ++j)
-            for (int64_t i = 0; i < NX; ++i) {
-                const int64_t c[3] = {i, j, k}, n[3] = {NX - 1, NY - 1, NZ - 1};
-                if (c[axis] == (hi ? n[axis] : 0)) {
-                    const int64_t id = i + NX * (j + NY * k);
-                    fixed[3 * id] = fixed[3 * id + 1] = fixed[3 * id + 2] = 1;
-                }
-            }
+    const int64_t n[3] = {NX - 1, NY - 1, NZ - 1};
+    const int ax1 = (axis + 1) % 3, ax2 = (axis + 2) % 3;
+    // the nodes with coordinate axis == face value
+    for (int64_t q = 0; q <= n[ax2]; ++q)
+        for (int64_t p = 0; p <= n[ax1]; ++p) {
+            int64_t c[3];
+            c[axis] = hi ? n[axis] : 0;
+            c[ax1] = p;
+            c[ax2] = q;
+            const int64_t id = c[0] + NX * (c[1] + NY * c[2]);
+            fixed[3 * id] = fixed[3 * id + 1] = fixed[3 * id + 2] = 1;
+        }
+    std::map<int64_t, double> f;
     const int nl = std::max(1, std::min(4, d.n_loads));
     for (int l = 0; l < nl; ++l) {
         const double a = unif(d.seed, 4 * l), b = unif(d.seed, 4 * l + 1), z = 2.0 * unif(d.seed, 4 * l + 2) - 1.0,
                      phi = 2.0 * M_PI * unif(d.seed, 4 * l + 3);
         int64_t c[3];
-        const int64_t n[3] = {NX - 1, NY - 1, NZ - 1};
-        const int ax1 = (axis + 1) % 3, ax2 = (axis + 2) % 3;
         c[axis] = hi ? 0 : n[axis];   // opposite face
         c[ax1] = std::min<int64_t>(n[ax1], (int64_t)(a * (n[ax1] + 1)));
         c[ax2] = std::min<int64_t>(n[ax2], (int64_t)(b * (n[ax2] + 1)));
@@ -115,6 +118,12 @@ void make_problem(const hexa_job& d, std::vector<uint8_t>& fixed, std::vector<do
         f[3 * id] += s * std::cos(phi);
         f[3 * id + 1] += s * std::sin(phi);
         f[3 * id + 2] += z;
+    }
+    fidx.clear();
+    fval.clear();
+    for (const auto& kv : f) {
+        fidx.push_back(kv.first);
+        fval.push_back(kv.second);
     }
 }
 
@@ -131,11 +140,11 @@ int32_t run_job(Job* jb) {
     int32_t st = hexa_create(&g, &o, &h);
     if (st != HEXA_OK) return st;
     std::vector<uint8_t> fixed;
-    std::vector<double> f;
-    make_problem(d, fixed, f);
-    st = hexa_set_problem(h, fixed.data(), f.data(), nullptr);
+    std::vector<int64_t> fidx;
+    std::vector<double> fval;
+    make_problem(d, fixed, fidx, fval);
+    st = set_problem_impl(h, fixed.data(), nullptr, (int64_t)fidx.size(), fidx.data(), fval.data(), nullptr);
     const int64_t ne = (int64_t)d.nx * d.ny * d.nz;
-    jb->x.assign(ne, d.volfrac);
     hexa_oc_params p;
     std::memset(&p, 0, sizeof p);
     p.volfrac = d.volfrac;
@@ -149,14 +158,28 @@ int32_t run_job(Job* jb) {
     p.solve.precond = HEXA_PRECOND_MG;
     p.solve.semantics = HEXA_SEMANTICS_CPU;
     p.solve.max_iter = 2000;
+    // the design stays on the device (the handle's OC design array) from the
+    // uniform start to the last iteration, then comes back once
+    if (st == HEXA_OK) st = oc_check(h, &p);
+    if (st == HEXA_OK) {
+        launch_fill<double>(h->stream, ne, d.volfrac, h->d_e0);
+        st = check_launch();
+    }
     int solves = 0, iters = 0;
     hexa_oc_report rep{};
     for (int it = 0; it < d.oc_iters && st == HEXA_OK; ++it) {
         p.solve.warm_start = it > 0;
-        st = hexa_oc_step(h, jb->x.data(), &p, &rep);
+        st = oc_step_core(h, &p, &rep, now_ms());
         if (st == HEXA_E_NOT_CONVERGED) st = HEXA_OK;
         ++solves;
         iters += rep.solve_iterations;
+    }
+    jb->x.assign(ne, 0.0);
+    if (st == HEXA_OK &&
+        (hipMemcpyAsync(jb->x.data(), h->d_e0, sizeof(double) * ne, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess)) {
+        set_error("design download failed");
+        st = HEXA_E_DEVICE;
     }
     jb->res.oc_iters_done = solves;
     jb->res.solve_iterations = iters;

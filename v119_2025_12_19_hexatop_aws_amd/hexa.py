@@ -160,6 +160,7 @@ SIGNATURES = {
     "hexa_oc_update": (C.c_int32, [_H, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
                                    C.POINTER(C.c_double)]),
     "hexa_oc_step": (C.c_int32, [_H, _P, C.POINTER(OcParams), C.POINTER(OcReport)]),
+    "hexa_oc_step_dev": (C.c_int32, [_H, C.c_void_p, C.POINTER(OcParams), C.POINTER(OcReport)]),
     "hexa_update_density": (C.c_int32, [_H, _P, _P, _P, _P, C.POINTER(UpdateParams), C.POINTER(UpdateReport)]),
     "hexa_set_update_fields": (C.c_int32, [_H, _P, _P]),
     "hexa_write_vti": (C.c_int32, [_H, C.c_char_p, _P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
@@ -372,6 +373,16 @@ class Solver:
         st = lib().hexa_oc_step(self._h, _ptr(x), C.byref(p), C.byref(rep))
         _check(st, allow=(OK, E_NOT_CONVERGED))
         return x, rep.as_dict()
+
+    def oc_step_dev(self, d_x, volfrac, move, rmin, filter_tol=1e-10, filter_max_iter=500, bisect_tol=1e-12,
+                    **solve):
+        """oc_step on a design in device memory (d_x: device pointer to nE doubles, updated in place)."""
+        p = OcParams(volfrac=volfrac, move=move, rmin=rmin, filter_tol=filter_tol, bisect_tol=bisect_tol,
+                     filter_max_iter=filter_max_iter, solve=default_params(**solve))
+        rep = OcReport()
+        st = lib().hexa_oc_step_dev(self._h, C.c_void_p(int(d_x)), C.byref(p), C.byref(rep))
+        _check(st, allow=(OK, E_NOT_CONVERGED))
+        return rep.as_dict()
 
     def update_density(self, density, l1, protected, original, **params):
         d = np.array(density, dtype=np.float32).reshape(-1)
