@@ -40,6 +40,22 @@ def test_filter_matches_golden_and_oracle(bug):
     assert np.abs(o2 - oo).max() < 1e-10
 
 
+def test_filter_large_grid_matches_oracle():
+    """A grid large enough that each thread of the stencil kernel marches
+    several planes (256x128x64): random density against the oracle, and the
+    constant-field closed form (2/3 rho with the reference element)."""
+    say("filter large grid")
+    nx, ny, nz = 256, 128, 64
+    rho = np.random.default_rng(3).uniform(0.0, 1.0, nx * ny * nz)
+    with hexa.Solver(nx, ny, nz) as s:
+        out, it, rel = s.filter(rho, 2.0, tol=1e-8, max_iter=1000, check_every=1)
+        c, itc, relc = s.filter(np.full(nx * ny * nz, 0.6), 2.0, tol=1e-14, max_iter=1000, check_every=1)
+    ref, ito, relo = oracle.filter_density((nx, ny, nz), (1, 1, 1), 2.0, rho, tol=1e-8, max_iter=1000, check_every=1)
+    assert it == ito
+    assert np.abs(out - ref).max() < 1e-9
+    assert np.abs(c - 0.4).max() < 1e-12
+
+
 def test_filter_constant_and_zero_fields():
     say("filter closed forms")
     with hexa.Solver(12, 8, 6, (1.0, 1.0, 1.0)) as s:

@@ -22,7 +22,7 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     hipStream_t st = h->stream;
     const Grid3& g = h->g;
     const int64_t nn = g.nn;
-    HX_TRY(dalloc(h, &h->d_Ae, 64));
+    HX_TRY(dalloc(h, &h->d_Ae, 64 + 27));
     HX_TRY(dalloc(h, &h->d_hb, (size_t)nn));
     HX_TRY(dalloc(h, &h->d_hx, (size_t)nn));
     HX_TRY(dalloc(h, &h->d_hr, (size_t)nn));
@@ -32,9 +32,10 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
     HX_TRY(dalloc(h, &h->d_scal, 16));
     HX_TRY(dalloc(h, &h->d_state, 1));
-    double K8[64], M8[64], Ae[64];
+    double K8[64], M8[64], Ae[64 + 27];
     helmholtz_element(h->grid.dx, h->grid.dy, h->grid.dz, h->opt.helm_element == HEXA_HELM_BUG_COMPAT, K8, M8);
     for (int i = 0; i < 64; ++i) Ae[i] = radius * radius * K8[i] + M8[i];
+    helm_stencil(Ae, Ae + 64);
     HX_CHECK(hipMemcpyAsync(h->d_Ae, Ae, sizeof Ae, hipMemcpyHostToDevice, st));
     const double w = h->grid.dx * h->grid.dy * h->grid.dz / 8.0;
     launch_helm_rhs_diag(st, g, h->d_Ae, d_in, w, h->d_hb, h->d_hdinv);

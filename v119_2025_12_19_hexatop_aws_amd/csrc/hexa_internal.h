@@ -198,6 +198,9 @@ void launch_axpy(hipStream_t s, int64_t n, T a, const T* x, T* y);
 
 // Helmholtz filter (k_filter.hip)
 void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const double* x, double* y);
+// the 27-point stencil of an interior node from the 8x8 element matrix
+// (written after Ae's 64 values for launch_helm_apply)
+void helm_stencil(const double* Ae, double* S);
 void launch_helm_rhs_diag(hipStream_t s, const Grid3& g, const double* Ae, const double* rho, double w, double* b,
                           double* dinv);
 void launch_helm_extract(hipStream_t s, const Grid3& g, const double* x, double scale, double* out);

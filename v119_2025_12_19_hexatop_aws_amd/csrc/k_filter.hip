@@ -24,38 +24,64 @@ __device__ constexpr int FCZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 __device__ constexpr int fcorner(int cx, int cy, int cz) { return 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0)); }
 
 // y = A x, A = sum_e Ae over the elements of the box (scalar DOF per node).
+// Ae holds the 8x8 element matrix (64 values) followed by the 27-point stencil
+// of an interior node (27 values, offsets (dx,dy,dz) at dx+1 + 3(dy+1) +
+// 9(dz+1)): A has the same coefficients wherever all 8 elements around a node
+// exist, so interior nodes take 27 FMAs; nodes on the box faces sum their
+// existing elements' rows.  A thread marches a chunk of planes along z with
+// the 3x3 neighbourhoods of planes k-1, k, k+1 in registers, loading 9 values
+// per node (neighbours in x and y come from cache).
 __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __restrict__ Ae,
-                                                    const double* __restrict__ x, double* __restrict__ y) {
+                                                    const double* __restrict__ x, double* __restrict__ y,
+                                                    int kchunk) {
     const int NX = g.nx + 1, NY = g.ny + 1, NZ = g.nz + 1;
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    const int k0 = blockIdx.z * kchunk, k1 = min(k0 + kchunk, NZ);
     if (i >= NX || j >= NY) return;
-    double xn[27];
-#pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
+    const double* S = Ae + 64;
+    const bool inxy = i >= 1 && i < g.nx && j >= 1 && j < g.ny;
+    auto load = [&](int kk, double (&p)[9]) {
+        const bool kin = (unsigned)kk < (unsigned)NZ;
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
-                const int ii = i + dx - 1, jj = j + dy - 1, kk = k + dz - 1;
-                const bool in = (unsigned)ii < (unsigned)NX && (unsigned)jj < (unsigned)NY && (unsigned)kk < (unsigned)NZ;
-                xn[dx + 3 * dy + 9 * dz] = in ? x[ii + (int64_t)NX * (jj + (int64_t)NY * kk)] : 0.0;
+                const int ii = i + dx - 1, jj = j + dy - 1;
+                const bool in = kin && (unsigned)ii < (unsigned)NX && (unsigned)jj < (unsigned)NY;
+                p[dx + 3 * dy] = in ? x[ii + (int64_t)NX * (jj + (int64_t)NY * kk)] : 0.0;
             }
-    double acc = 0.0;
+    };
+    double w[3][9];
+    load(k0 - 1, w[0]);
+    load(k0, w[1]);
+    for (int k = k0; k < k1; ++k) {
+        load(k + 1, w[2]);
+        double acc = 0.0;
+        if (inxy && k >= 1 && k < g.nz) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const int ex = (s & 1) - 1, ey = ((s >> 1) & 1) - 1, ez = ((s >> 2) & 1) - 1;
-        const bool ok = (unsigned)(i + ex) < (unsigned)g.nx && (unsigned)(j + ey) < (unsigned)g.ny &&
-                        (unsigned)(k + ez) < (unsigned)g.nz;
-        const int l = fcorner(-ex, -ey, -ez);
-        double r = 0.0;
+            for (int q = 0; q < 27; ++q) acc += S[q] * w[q / 9][q % 9];
+        } else {
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
-            r += Ae[l * 8 + m] * xn[(ex + FCX[m] + 1) + 3 * (ey + FCY[m] + 1) + 9 * (ez + FCZ[m] + 1)];
-        acc += ok ? r : 0.0;
+            for (int s = 0; s < 8; ++s) {
+                const int ex = (s & 1) - 1, ey = ((s >> 1) & 1) - 1, ez = ((s >> 2) & 1) - 1;
+                const bool ok = (unsigned)(i + ex) < (unsigned)g.nx && (unsigned)(j + ey) < (unsigned)g.ny &&
+                                (unsigned)(k + ez) < (unsigned)g.nz;
+                const int l = fcorner(-ex, -ey, -ez);
+                double r = 0.0;
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                    r += Ae[l * 8 + m] * w[ez + FCZ[m] + 1][(ex + FCX[m] + 1) + 3 * (ey + FCY[m] + 1)];
+                acc += ok ? r : 0.0;
+            }
+        }
+        y[i + (int64_t)NX * (j + (int64_t)NY * k)] = acc;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            w[0][q] = w[1][q];
+            w[1][q] = w[2][q];
+        }
     }
-    y[i + (int64_t)NX * (j + (int64_t)NY * k)] = acc;
 }
 
 // b_n = w * sum of rho over the elements around n; diag_n = sum of Ae[l][l].
@@ -97,9 +123,31 @@ __global__ void k_helm_extract(Grid3 g, const double* __restrict__ x, double sca
 }  // namespace
 
 void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const double* x, double* y) {
-    dim3 block(64, 4, 1);
-    dim3 grid((unsigned)((g.nx + 1 + 63) / 64), (unsigned)((g.ny + 1 + 3) / 4), (unsigned)(g.nz + 1));
-    k_helm_apply<<<grid, block, 0, s>>>(g, Ae, x, y);
+    const dim3 block(64, 4, 1);
+    const unsigned gx = (unsigned)((g.nx + 1 + 63) / 64), gy = (unsigned)((g.ny + 1 + 3) / 4);
+    const int NZ = g.nz + 1;
+    int kchunk = 16;   // planes per thread; fewer when that leaves too few blocks
+    while (kchunk > 1 && (int64_t)gx * gy * ((NZ + kchunk - 1) / kchunk) < 2048) kchunk >>= 1;
+    const dim3 grid(gx, gy, (unsigned)((NZ + kchunk - 1) / kchunk));
+    k_helm_apply<<<grid, block, 0, s>>>(g, Ae, x, y, kchunk);
+}
+
+void helm_stencil(const double* Ae, double* S) {
+    for (int q = 0; q < 27; ++q) S[q] = 0.0;
+    // element (ex,ey,ez) in {-1,0}^3 around the node; the node is its corner
+    // (-ex,-ey,-ez), the neighbour at offset (dx,dy,dz) its corner (dx-ex, ...)
+    for (int s = 0; s < 8; ++s) {
+        const int ex = (s & 1) - 1, ey = ((s >> 1) & 1) - 1, ez = ((s >> 2) & 1) - 1;
+        const int l = 4 * (-ez) + ((-ey) ? ((-ex) ? 2 : 3) : ((-ex) ? 1 : 0));
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int cx = dx - ex, cy = dy - ey, cz = dz - ez;
+                    if (cx < 0 || cx > 1 || cy < 0 || cy > 1 || cz < 0 || cz > 1) continue;
+                    const int m = 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0));
+                    S[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] += Ae[l * 8 + m];
+                }
+    }
 }
 void launch_helm_rhs_diag(hipStream_t s, const Grid3& g, const double* Ae, const double* rho, double w, double* b,
                           double* dinv) {
