@@ -6,7 +6,6 @@ on grids with several tiles along x (nx > 62), several along y and several
 node-plane chunks along z.  The mixed V-cycle (FP32 sweeps, transfers and
 BF16 inverse diagonals, FP64 coarsest solve) is compared with the FP64
 V-cycle on the same residual, to the BF16 rounding of the diagonals."""
-import ctypes as C
 import os
 import sys
 
@@ -16,6 +15,7 @@ import pytest
 import oracle
 from v119_2025_12_19_hexatop_aws_amd import hexa
 from v119_2025_12_19_hexatop_aws_amd import problems as P
+from helpers import DevBuf, hip_runtime as _hip
 
 pytestmark = pytest.mark.gpu
 
@@ -25,37 +25,6 @@ def say(*a):
     sys.stdout.flush()
 
 
-def _hip():
-    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
-        try:
-            L = C.CDLL(name)
-            break
-        except OSError:
-            continue
-    L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    L.hipFree.argtypes = [C.c_void_p]
-    L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    L.hipDeviceSynchronize.argtypes = []
-    return L
-
-
-class DevBuf:
-    """A device buffer holding a copy of a host array."""
-
-    def __init__(self, hip, a):
-        self.hip, self.a = hip, a
-        self.p = C.c_void_p()
-        assert hip.hipMalloc(C.byref(self.p), a.nbytes) == 0
-        assert hip.hipMemcpy(self.p, a.ctypes.data_as(C.c_void_p), a.nbytes, 1) == 0
-
-    def read(self):
-        out = np.empty_like(self.a)
-        assert self.hip.hipDeviceSynchronize() == 0
-        assert self.hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.p, out.nbytes, 2) == 0
-        return out
-
-    def free(self):
-        self.hip.hipFree(self.p)
 
 
 GRIDS = [((33, 9, 5), (1.0, 1.0, 1.0)), ((130, 15, 9), (1.0, 0.5, 2.0)), ((64, 20, 17), (1.0, 1.0, 1.0)),

@@ -191,15 +191,16 @@ def test_oc_loop_matches_oracle_small(precision):
 def test_oc_step_dev_equals_host_oc_step():
     """hexa_oc_step_dev (design in device memory) gives bitwise the same
     designs and reports as hexa_oc_step over three iterations."""
-    import torch
+    from helpers import DevBuf, hip_runtime
     say("oc step dev")
+    hip = hip_runtime()
     p = P.cantilever(24, 8, 8, seed=2)
     kw = dict(filter_tol=1e-12, filter_max_iter=2000, bisect_tol=1e-12, tol=1e-9, precond=hexa.PRECOND_MG,
               semantics=hexa.SEMANTICS_CPU)
     out = {}
     for mode in ("host", "dev"):
         x = np.full(p.n_elem, 0.3)
-        dx = torch.from_numpy(x.copy()).cuda()
+        dx = DevBuf(hip, x.copy())
         comps = []
         with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0, precision=hexa.PREC_MIXED) as s:
             s.set_problem(p.fixed, p.f[0])
@@ -207,10 +208,10 @@ def test_oc_step_dev_equals_host_oc_step():
                 if mode == "host":
                     x, rep = s.oc_step(x, 0.3, 0.2, 1.5, warm_start=int(it > 0), **kw)
                 else:
-                    torch.cuda.synchronize()
-                    rep = s.oc_step_dev(dx.data_ptr(), 0.3, 0.2, 1.5, warm_start=int(it > 0), **kw)
+                    rep = s.oc_step_dev(dx.p.value, 0.3, 0.2, 1.5, warm_start=int(it > 0), **kw)
                 comps.append((rep["compliance"], rep["solve_iterations"], rep["filter_iterations"]))
-        out[mode] = (x if mode == "host" else dx.cpu().numpy(), comps)
+        out[mode] = (x if mode == "host" else dx.read(), comps)
+        dx.free()
     assert np.array_equal(out["host"][0], out["dev"][0])
     assert out["host"][1] == out["dev"][1]
 

@@ -18,6 +18,8 @@ inline unsigned grid_for(int64_t n, int threads = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 __device__ constexpr int FCX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 __device__ constexpr int FCY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __device__ constexpr int FCZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
@@ -29,8 +31,9 @@ __device__ constexpr int fcorner(int cx, int cy, int cz) { return 4 * cz + (cy ?
 // 9(dz+1)): A has the same coefficients wherever all 8 elements around a node
 // exist, so interior nodes take 27 FMAs; nodes on the box faces sum their
 // existing elements' rows.  A thread marches a chunk of planes along z with
-// the 3x3 neighbourhoods of planes k-1, k, k+1 in registers, loading 9 values
-// per node (neighbours in x and y come from cache).
+// the 3x3 neighbourhoods of planes k-1, k, k+1 in registers and those of
+// plane k+2 in flight, loading 9 values per node (neighbours in x and y come
+// from cache).
 __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __restrict__ Ae,
                                                     const double* __restrict__ x, double* __restrict__ y,
                                                     int kchunk) {
@@ -41,22 +44,33 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
     if (i >= NX || j >= NY) return;
     const double* S = Ae + 64;
     const bool inxy = i >= 1 && i < g.nx && j >= 1 && j < g.ny;
+    // the 9 loads of a plane go through a buffer descriptor of that plane:
+    // a neighbour outside the box is an out-of-range offset and reads 0, so
+    // the loads carry no per-lane condition (a conditional load compiles to
+    // a branch and a wait of its own, which serialises the round trips)
     auto load = [&](int kk, double (&p)[9]) {
         const bool kin = (unsigned)kk < (unsigned)NZ;
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double*>(x + (kin ? (int64_t)NX * NY * kk : 0)), (short)0, kin ? NX * NY * 8 : 0, 0x00020000);
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 const int ii = i + dx - 1, jj = j + dy - 1;
-                const bool in = kin && (unsigned)ii < (unsigned)NX && (unsigned)jj < (unsigned)NY;
-                p[dx + 3 * dy] = in ? x[ii + (int64_t)NX * (jj + (int64_t)NY * kk)] : 0.0;
+                const bool in = (unsigned)ii < (unsigned)NX && (unsigned)jj < (unsigned)NY;
+                const uint32_t off = in ? (uint32_t)(ii + NX * jj) * 8u : 0x80000000u;
+                p[dx + 3 * dy] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
             }
     };
-    double w[3][9];
-    load(k0 - 1, w[0]);
-    load(k0, w[1]);
-    for (int k = k0; k < k1; ++k) {
-        load(k + 1, w[2]);
+    // Step at plane k: a, b, c hold planes k-1, k, k+1 and d receives plane
+    // k+2, in flight while plane k is computed.  Four steps rotate the four
+    // buffers back to their roles, so no register is copied across the loop
+    // (a copy of an in-flight load would make the loop wait for it).  Planes
+    // past the chunk are loaded and computed but not stored: their stores
+    // go through a descriptor of zero size.
+    auto step = [&](const double (&a)[9], const double (&b)[9], const double (&c)[9], double (&d)[9], int k) {
+        load(k + 2, d);
+        const double* w[3] = {a, b, c};
         double acc = 0.0;
         if (inxy && k >= 1 && k < g.nz) {
 #pragma unroll
@@ -75,12 +89,21 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
                 acc += ok ? r : 0.0;
             }
         }
-        y[i + (int64_t)NX * (j + (int64_t)NY * k)] = acc;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            w[0][q] = w[1][q];
-            w[1][q] = w[2][q];
-        }
+        const bool kst = k < k1;
+        const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(
+            y + (kst ? (int64_t)NX * NY * k : 0), (short)0, kst ? NX * NY * 8 : 0, 0x00020000);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, acc), ry, (uint32_t)(i + NX * j) * 8u, 0,
+                                              0);
+    };
+    double wa[9], wb[9], wc[9], wd[9];
+    load(k0 - 1, wa);
+    load(k0, wb);
+    load(k0 + 1, wc);
+    for (int k = k0; k < k1; k += 4) {
+        step(wa, wb, wc, wd, k);
+        step(wb, wc, wd, wa, k + 1);
+        step(wc, wd, wa, wb, k + 2);
+        step(wd, wa, wb, wc, k + 3);
     }
 }
 
