@@ -47,6 +47,38 @@ __device__ __forceinline__ V3<T> ld(const T* p, int64_t node) { return *reinterp
 template <typename T>
 __device__ __forceinline__ void st(T* p, int64_t node, const V3<T>& v) { *reinterpret_cast<V3<T>*>(p + 3 * node) = v; }
 
+// Loads through buffer descriptors: an offset past the descriptor's size
+// (OOB) or a descriptor of size 0 reads zeros, so the transfers' neighbour
+// loads carry no per-lane condition.  (A load under a per-lane condition
+// compiles to a branch and a wait of its own, which serialises the round
+// trips of a thread's loads.)  A descriptor covers one node plane (at most a
+// few MB, so 32-bit offsets suffice for any grid).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t OOB = 0x80000000u;
+__device__ __forceinline__ Rsrc rsrc(const void* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+// the descriptor of node plane k of a 3-component node array (size 0 when
+// k is outside [0, nz))
+template <typename T>
+__device__ __forceinline__ Rsrc plane3(const T* p, int k, int nz, int plane_nodes) {
+    const bool in = (unsigned)k < (unsigned)nz;
+    return rsrc(p + (in ? 3 * (int64_t)plane_nodes * k : 0), in ? plane_nodes * 3 * (int)sizeof(T) : 0);
+}
+__device__ __forceinline__ V3<float> ldb(Rsrc r, uint32_t off, float) {
+    const u32x3 a = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+    return V3<float>{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
+}
+__device__ __forceinline__ V3<double> ldb(Rsrc r, uint32_t off, double) {
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0);
+    return V3<double>{__builtin_bit_cast(double, u32x2{a.x, a.y}), __builtin_bit_cast(double, u32x2{a.z, a.w}),
+                      __builtin_bit_cast(double, b)};
+}
+
 // r_c(I,J,K) = sum over the 3x3x3 fine nodes around (2I,2J,2K) of
 // w(di) w(dj) w(dk) r_f, w(0) = 1, w(+-1) = 1/2; held coarse DOFs -> 0.
 template <typename T>
@@ -60,21 +92,20 @@ __global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, FastDiv dnx, 
     const int FNX = f.nx + 1, FNY = f.ny + 1, FNZ = f.nz + 1;
     const uint8_t mb = cmask[n];
     V3<T> s{T(0), T(0), T(0)};
+#pragma unroll
     for (int dk = -1; dk <= 1; ++dk) {
-        const int k = 2 * (int)K + dk;
-        if ((unsigned)k >= (unsigned)FNZ) continue;
+        const Rsrc rp = plane3(rf, 2 * (int)K + dk, FNZ, FNX * FNY);
         const T wk = dk == 0 ? T(1) : T(0.5);
+#pragma unroll
         for (int dj = -1; dj <= 1; ++dj) {
             const int j = 2 * J + dj;
-            if ((unsigned)j >= (unsigned)FNY) continue;
             const T wjk = wk * (dj == 0 ? T(1) : T(0.5));
-            const int64_t frow = (int64_t)FNX * (j + (int64_t)FNY * k);
 #pragma unroll
             for (int di = -1; di <= 1; ++di) {
                 const int i = 2 * I + di;
-                if ((unsigned)i >= (unsigned)FNX) continue;
+                const bool in = (unsigned)j < (unsigned)FNY && (unsigned)i < (unsigned)FNX;
                 const T w = wjk * (di == 0 ? T(1) : T(0.5));
-                const V3<T> v = ld(rf, frow + i);
+                const V3<T> v = ldb(rp, in ? (uint32_t)(i + FNX * j) * (3 * sizeof(T)) : OOB, T(0));
                 s.x += w * v.x;
                 s.y += w * v.y;
                 s.z += w * v.z;
@@ -106,29 +137,29 @@ __global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, FastDiv dp
     const uint8_t mb0 = fmask[n], mb1 = odd_in ? fmask[n + 1] : (uint8_t)7;
     if ((mb0 & 7) == 7 && (mb1 & 7) == 7) return;
     const int j0 = j >> 1, k0 = (int)k >> 1;
-    const int nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-    const bool c1_in = m + 1 < CNX;                       // coarse column m+1 exists
-    V3<T> s0{T(0), T(0), T(0)}, s1{T(0), T(0), T(0)};    // coarse column m, m+1 sums
-    for (int a = 0; a < nk; ++a) {
-        if (k0 + a >= CNZ) continue;
-        for (int bb = 0; bb < nj; ++bb) {
-            if (j0 + bb >= CNY) continue;
-            const int64_t crow = (int64_t)CNX * ((j0 + bb) + (int64_t)CNY * (k0 + a));
-            if (m < CNX) {
-                const V3<T> v = ld(xc, crow + m);
-                s0.x += v.x;
-                s0.y += v.y;
-                s0.z += v.z;
-            }
-            if (c1_in) {
-                const V3<T> v = ld(xc, crow + m + 1);
-                s1.x += v.x;
-                s1.y += v.y;
-                s1.z += v.z;
-            }
+    const bool jodd = j & 1, kodd = k & 1;
+    // coarse columns m and m+1 of rows j0 (and j0+1 when j is odd) of planes
+    // k0 (and k0+1 when k is odd): all 8 loads issued, those outside the
+    // coarse grid or not needed read 0
+    V3<T> s0{T(0), T(0), T(0)}, s1{T(0), T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const Rsrc rp = plane3(xc, (a == 0 || kodd) ? k0 + a : -1, CNZ, CNX * CNY);
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+            const bool rin = (bb == 0 || jodd) && j0 + bb < CNY;
+            const uint32_t roff = (uint32_t)(CNX * (j0 + bb) + m) * (3 * sizeof(T));
+            const V3<T> v0 = ldb(rp, rin && m < CNX ? roff : OOB, T(0));
+            const V3<T> v1 = ldb(rp, rin && m + 1 < CNX ? roff + 3 * sizeof(T) : OOB, T(0));
+            s0.x += v0.x;
+            s0.y += v0.y;
+            s0.z += v0.z;
+            s1.x += v1.x;
+            s1.y += v1.y;
+            s1.z += v1.z;
         }
     }
-    const T wyz = (nj == 2 ? T(0.5) : T(1)) * (nk == 2 ? T(0.5) : T(1));
+    const T wyz = (jodd ? T(0.5) : T(1)) * (kodd ? T(0.5) : T(1));
     if ((mb0 & 7) != 7) {
         V3<T> x = ld(xf, n);
         if (!(mb0 & 1)) x.x += wyz * s0.x;
