@@ -53,6 +53,27 @@ __device__ __forceinline__ T to_value(DT v) { return (T)v; }
 template <>
 __device__ __forceinline__ float to_value<float, uint16_t>(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
 
+// Loads through buffer descriptors of one plane: an offset past the
+// descriptor's size (OOB), or a descriptor of size 0, reads 0, so the
+// kernel's loads carry no per-lane condition (a guarded load compiles to a
+// branch and a wait of its own).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t OOB = 0x80000000u;
+__device__ __forceinline__ Rsrc plane_rsrc(const void* base, int64_t plane_bytes, int k, int nk) {
+    const bool in = (unsigned)k < (unsigned)nk;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(static_cast<const char*>(base) + (in ? plane_bytes * k : 0)),
+                                             (short)0, in ? (int)plane_bytes : 0, 0x00020000);
+}
+__device__ __forceinline__ double bload(Rsrc r, uint32_t off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float bload(Rsrc r, uint32_t off, float) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ uint16_t bload(Rsrc r, uint32_t off, uint16_t) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+
 constexpr int MX = 16;              // elements per group (MFMA N) and per tile row
 constexpr int MY = 16;              // groups (element rows) per tile
 constexpr int OWN = 15;             // owned nodes per tile axis
@@ -108,18 +129,17 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const T* __restric
     }
 
     // staging of node plane kp into LDS slot: items tid and tid + NT
+    const int64_t nplane = (int64_t)NX * NY;   // nodes per plane
     auto stage_load = [&](int kp, T (&v)[2]) {
+        const Rsrc r = plane_rsrc(u, nplane * 3 * (int64_t)sizeof(T), kp, NZ);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int it = tid + q * NT;
-            v[q] = T(0);
-            if (it < PLANE) {
-                const int r = it / PROW, w = it - r * PROW;
-                const int nx = w / 3, comp = w - 3 * nx;
-                const int i = i0 + nx, j = j0 + r;
-                if ((unsigned)i < (unsigned)NX && (unsigned)j < (unsigned)NY && (unsigned)kp < (unsigned)NZ)
-                    v[q] = u[3 * (i + (int64_t)NX * (j + (int64_t)NY * kp)) + comp];
-            }
+            const int rr = it / PROW, w = it - rr * PROW;
+            const int nx = w / 3, comp = w - 3 * nx;
+            const int i = i0 + nx, j = j0 + rr;
+            const bool in = it < PLANE && (unsigned)i < (unsigned)NX && (unsigned)j < (unsigned)NY;
+            v[q] = bload(r, in ? (uint32_t)((i + NX * j) * 3 + comp) * (uint32_t)sizeof(T) : OOB, T(0));
         }
     };
     auto stage_store = [&](int slot, const T (&v)[2]) {
@@ -163,26 +183,31 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const T* __restric
         stage_load(ek + 2, nxt);
         // element factors of this wave's two groups (0 outside the grid)
         T f[2];
+        const Rsrc rf = plane_rsrc(fac, (int64_t)g.nx * g.ny * (int64_t)sizeof(T), ek, g.nz);
 #pragma unroll
         for (int gq = 0; gq < 2; ++gq) {
             const int ei = i0 + lj, ej = j0 + 2 * wave + gq;
-            const bool in = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny && (unsigned)ek < (unsigned)g.nz;
-            f[gq] = in ? fac[ei + (int64_t)g.nx * (ej + (int64_t)g.ny * ek)] : T(0);
+            const bool in = (unsigned)ei < (unsigned)g.nx && (unsigned)ej < (unsigned)g.ny;
+            f[gq] = bload(rf, in ? (uint32_t)(ei + g.nx * ej) * (uint32_t)sizeof(T) : OOB, T(0));
         }
         // flag, right-hand side and inverse diagonal of the items of output plane ek
         uint8_t mb[2] = {0, 0};
         T bv[2] = {T(0), T(0)}, dv[2] = {T(0), T(0)};
         int64_t gdof[2];
+        const int ko = ek >= k0 ? ek : -1;   // no output plane in the chunk's first layer
+        const Rsrc rm = plane_rsrc(mask, mask ? nplane : 0, ko, NZ);
+        const Rsrc rb = plane_rsrc(b, MODE != OP_APPLY ? nplane * 3 * (int64_t)sizeof(T) : 0, ko, NZ);
+        const Rsrc rd = plane_rsrc(dinv, MODE == OP_SMOOTH ? nplane * 3 * (int64_t)sizeof(DT) : 0, ko, NZ);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const bool o = gq_in[q] && ek >= k0;
-            const int64_t gn = gq_base[q] + (int64_t)NX * NY * (o ? ek : 0);
+            const int64_t gn = gq_base[q] + nplane * (ko >= 0 ? ko : 0);
             gdof[q] = 3 * gn + gq_c[q];
-            if (o) {
-                mb[q] = mask ? (mask[gn] >> gq_c[q]) & 1 : 0;
-                if (MODE != OP_APPLY) bv[q] = b[gdof[q]];
-                if (MODE == OP_SMOOTH) dv[q] = to_value<T, DT>(dinv[gdof[q]]);
-            }
+            const uint32_t pn = (uint32_t)gq_base[q];   // node index within the plane
+            mb[q] = (uint8_t)((__builtin_amdgcn_raw_buffer_load_b8(rm, gq_in[q] ? pn : OOB, 0, 0) >> gq_c[q]) & 1);
+            const uint32_t dof = 3 * pn + gq_c[q];
+            if (MODE != OP_APPLY) bv[q] = bload(rb, gq_in[q] ? dof * (uint32_t)sizeof(T) : OOB, T(0));
+            if (MODE == OP_SMOOTH)
+                dv[q] = to_value<T, DT>(bload(rd, gq_in[q] ? dof * (uint32_t)sizeof(DT) : OOB, DT(0)));
         }
 
         // Y = Ke U for the wave's two groups
