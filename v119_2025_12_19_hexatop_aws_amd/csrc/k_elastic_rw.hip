@@ -471,7 +471,7 @@ RwShape rw_shape(const Grid3& g, int kind = TK_RW64, int kchunk = 0) {
     r.kchunk = kchunk ? kchunk : tuned_chunk(g, kind);
     if (!r.kchunk) {
         r.kchunk = 32;
-        while (r.kchunk > 4 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
+        while (r.kchunk > 1 && (int64_t)r.tiles_x * r.tiles_y * ((g.nz + 1 + r.kchunk - 1) / r.kchunk) < 2048)
             r.kchunk /= 2;
     }
     r.tiles_z = (g.nz + 1 + r.kchunk - 1) / r.kchunk;
@@ -483,9 +483,9 @@ bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
     return 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
 }
 
-// (the shortest chunks the tuning tries: the most blocks)
+// (chunks of one plane: the most blocks any choice launches)
 int64_t elastic_rw_blocks(const Grid3& g) {
-    const RwShape r = rw_shape(g, TK_RW64, 4);
+    const RwShape r = rw_shape(g, TK_RW64, 1);
     return (int64_t)r.tiles_x * r.tiles_y * r.tiles_z;
 }
 
@@ -543,8 +543,12 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         }
     }
     if (!tuned_chunk(g, kind)) {
-        const int cand[4] = {32, 16, 8, 4};
-        float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        // chunks of 2 and 1 planes are tried only where chunks of 4 leave
+        // fewer than 4096 blocks (mid-size grids, latency-bound)
+        const int cand[6] = {32, 16, 8, 4, 2, 1};
+        const RwShape r4 = rw_shape(g, kind, 4);
+        const int NC = (int64_t)r4.tiles_x * r4.tiles_y * r4.tiles_z < 4096 ? 6 : 4;
+        float t[6] = {};
         auto launch = [&](int kc) {
             const RwShape r = rw_shape(g, kind, kc);
             const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
@@ -557,8 +561,8 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         for (int w = 0; w < 8; ++w) launch(cand[0]);
         bool ok = true;
         for (int pass = 0; pass < 2 && ok; ++pass)
-            for (int q = 0; q < 4 && ok; ++q) {
-                const int c = pass ? 3 - q : q;
+            for (int q = 0; q < NC && ok; ++q) {
+                const int c = pass ? NC - 1 - q : q;
                 launch(cand[c]);
                 (void)hipEventRecord(e0, s);
                 for (int rep = 0; rep < 3; ++rep) launch(cand[c]);
@@ -568,7 +572,7 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
                 if (ok && (pass == 0 || ms < t[c])) t[c] = ms;
             }
         int best = 0;
-        for (int c = 0; ok && c < 4; ++c)
+        for (int c = 0; ok && c < NC; ++c)
             if (!best || t[c] < t[best - 1]) best = c + 1;
         best = ok && best ? cand[best - 1] : 0;
         if (best) {
