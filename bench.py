@@ -48,7 +48,7 @@ def build_problem(cfg):
     return P.cantilever_c1()
 
 
-def cpu_baseline(p, sample_matvecs=2):
+def cpu_baseline(p, sample_matvecs=10):
     """CPU oracle matvec (CPUSolver.apply_stiffness restated) on the host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
