@@ -98,3 +98,25 @@ def test_c3_scale_operator_and_solves():
         assert st == hexa.OK and rmx[0]["true_rel_res"] < 1.01e-8
     assert rel(umx[0], u64[0]) < 1e-6
     assert abs(rmx[0]["compliance"] - r64[0]["compliance"]) < 1e-9 * abs(r64[0]["compliance"])
+
+
+def test_c3_scale_dense_matrix_core_operator():
+    """512x256x256 with the reference's f32-widened Ke (the matrix-core
+    kernel): symmetric to the widened Ke's own asymmetry (its Float32
+    integration leaves |Ke - Ke^T| <= 7.5e-9 against entries of 0.24, so
+    1e-7 relative; a transposed operand or wrong neighbour would be O(1)),
+    and within the Ke widening (2e-6 relative) of the factored product with
+    the exact brick Ke on the same vector."""
+    say("c3 dense")
+    p = P.cantilever_c3()
+    x = np.random.default_rng(4).standard_normal(p.n_dof)
+    with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0, ke_mode=hexa.KE_F32_WIDENED) as s:
+        s.set_problem(p.fixed, p.f[0])
+        s.set_density(p.rho)
+        assert _symmetry(s, p.n_dof, 3) < 1e-7
+        yw = s.matvec(x)
+    with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0) as s:
+        s.set_problem(p.fixed, p.f[0])
+        s.set_density(p.rho)
+        ye = s.matvec(x)
+    assert np.all(np.isfinite(yw)) and rel(yw, ye) < 2e-6
