@@ -18,7 +18,7 @@ namespace hexa {
 // residual checked every check_every iterations; d_out = element means of x.
 // R/src/Optimization/GPUHelmholtz.jl:190-266.
 int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double radius, double tol, int max_iter,
-                   int check_every, int* iters, double* rel) {
+                   int check_every, int* iters, double* rel, bool fused) {
     hipStream_t st = h->stream;
     const Grid3& g = h->g;
     const int64_t nn = g.nn;
@@ -60,7 +60,11 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     launch_pcg_set_rz(st, h->d_state, h->d_partials);
     int k_done = 0;
     bool stop = false;
+    fused = fused && check_every == 1;
     auto iteration = [&]() -> int32_t {
+        if (fused && launch_filter_fused_iteration(st, g, h->d_Ae, h->d_state, h->d_hx, h->d_hp, h->d_hr, h->d_hAp,
+                                                   h->d_hdinv, h->d_partials) == 0)
+            return check_launch();
         launch_helm_apply(st, g, h->d_Ae, h->d_hp, h->d_hAp);
         launch_pcg_pAp(st, nn, h->d_state, h->d_hp, h->d_hAp, h->d_partials);
         launch_pcg_alpha(st, h->d_state, h->d_partials, 0);
@@ -73,7 +77,7 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     // the handle's own, so only the check period keys it)
     const char* ge = getenv("HEXA_GRAPHS");
     const bool graphs = !(ge && ge[0] == '0');
-    if (graphs && !(h->filt_exec && h->filt_every == check_every)) {
+    if (graphs && !(h->filt_exec && h->filt_every == check_every && h->filt_fused == fused)) {
         CaptureGuard lock;
         if (h->filt_exec) {
             HX_CHECK(hipGraphExecDestroy(h->filt_exec));
@@ -97,6 +101,7 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
             return HEXA_E_DEVICE;
         }
         h->filt_every = check_every;
+        h->filt_fused = fused;
     }
     for (int k = 1; k <= max_iter && !stop; ++k) {
         if (graphs) HX_CHECK(hipGraphLaunch(h->filt_exec, st));
@@ -142,7 +147,7 @@ int32_t filter_weights(hexa_handle* h, double radius, double tol, int max_iter) 
     launch_fill<double>(h->stream, h->g.ne, 1.0, h->d_ones);
     int it;
     double rel;
-    HX_TRY(filter_dev(h, h->d_ones, h->d_w, radius, tol, max_iter, 1, &it, &rel));
+    HX_TRY(filter_dev(h, h->d_ones, h->d_w, radius, tol, max_iter, 1, &it, &rel, true));
     h->w_radius = radius;
     return HEXA_OK;
 }
@@ -300,7 +305,7 @@ int32_t oc_step_core(hexa_handle* h, const hexa_oc_params* p, hexa_oc_report* re
     // physical density x~ = F(x)
     int fit = 0;
     double frel = 0.0;
-    HX_TRY(filter_dev(h, h->d_e0, h->d_rho, p->rmin, p->filter_tol, p->filter_max_iter, 1, &fit, &frel));
+    HX_TRY(filter_dev(h, h->d_e0, h->d_rho, p->rmin, p->filter_tol, p->filter_max_iter, 1, &fit, &frel, true));
     R.filter_iterations = fit;
     HX_TRY(update_factor(h));
     // K(x~) u = f
@@ -318,7 +323,7 @@ int32_t oc_step_core(hexa_handle* h, const hexa_oc_params* p, hexa_oc_report* re
                      h->d_partials);
     launch_final_sum(st, 1, h->d_partials, h->d_scal + 4);
     HX_TRY(check_launch());
-    HX_TRY(filter_dev(h, h->d_e2, h->d_e3, p->rmin, p->filter_tol, p->filter_max_iter, 1, &fit, &frel));
+    HX_TRY(filter_dev(h, h->d_e2, h->d_e3, p->rmin, p->filter_tol, p->filter_max_iter, 1, &fit, &frel, true));
     HX_TRY(filter_weights(h, p->rmin, p->filter_tol, p->filter_max_iter));
     launch_oc(st, ne, h->d_e3, h->d_w, h->d_w, p->volfrac, p->move, p->bisect_tol, 48, h->d_e0, h->d_e4, h->d_e5,
               h->d_e6, h->d_bracket, h->d_partials, h->d_scal);

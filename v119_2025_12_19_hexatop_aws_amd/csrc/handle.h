@@ -56,6 +56,7 @@ struct hexa_handle {
     int64_t graph_gen = 0;
     hipGraphExec_t filt_exec = nullptr;   // one Helmholtz-filter PCG iteration
     int filt_every = 0;
+    bool filt_fused = false;
 
     double Ke[576];
     bool factored = false;          // exact brick Ke -> factored element kernel
@@ -138,7 +139,9 @@ int32_t ensure_solver_vectors(hexa_handle* h);
 // reports whether it halted; records iteration k's state copy.
 int32_t poll_state(hexa_handle* h, int k, bool* halted);
 // Helmholtz filter of an element field on the handle's grid (device arrays).
+// fused: the three-launch iteration of launch_filter_fused_iteration
+// (residual checked every iteration; check_every must be 1).
 int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double radius, double tol, int max_iter,
-                   int check_every, int* iters, double* rel);
+                   int check_every, int* iters, double* rel, bool fused = false);
 
 }  // namespace hexa

@@ -113,6 +113,7 @@ struct PcgState {
     int32_t x_iter;    // iteration whose x update k_update deferred
     int32_t k;         // iterations started (advanced by the alpha kernels)
     int32_t x_done;    // last iteration whose deferred x update was applied
+    double rzb[2];     // (r, z) by iteration parity (the fused filter PCG)
 };
 
 // ---- kernel launchers (implemented in the .hip files) -----------------
@@ -198,6 +199,13 @@ void launch_axpy(hipStream_t s, int64_t n, T a, const T* x, T* y);
 
 // Helmholtz filter (k_filter.hip)
 void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const double* x, double* y);
+// One iteration of the fused filter PCG (three launches; state and vectors
+// as the filter's PCG, partials >= FUSED_PAP_MAX + 512 values); returns -1
+// when the grid needs more stencil blocks than FUSED_PAP_MAX.
+constexpr int FUSED_PAP_MAX = 16384;
+static_assert(FUSED_PAP_MAX + 512 <= MAX_RED_Q * RED_BLOCKS, "fused filter partials exceed the reduction buffer");
+int launch_filter_fused_iteration(hipStream_t s, const Grid3& g, const double* Ae, PcgState* st, double* x,
+                                  double* p, double* r, double* Ap, const double* dinv, double* partials);
 // the 27-point stencil of an interior node from the 8x8 element matrix
 // (written after Ae's 64 values for launch_helm_apply)
 void helm_stencil(const double* Ae, double* S);

@@ -34,15 +34,24 @@ __device__ constexpr int fcorner(int cx, int cy, int cz) { return 4 * cz + (cy ?
 // the 3x3 neighbourhoods of planes k-1, k, k+1 in registers and those of
 // plane k+2 in flight, loading 9 values per node (neighbours in x and y come
 // from cache).
+// With PAP (the fused filter PCG), a block also writes its partial sum of
+// x.y over the nodes it computes to pap[block] and block 0 advances the
+// iteration count; a halted state skips the launch.
+template <bool PAP>
 __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __restrict__ Ae,
                                                     const double* __restrict__ x, double* __restrict__ y,
-                                                    int kchunk) {
+                                                    int kchunk, PcgState* st, double* __restrict__ pap) {
+    if (PAP && st->halted) return;
     const int NX = g.nx + 1, NY = g.ny + 1, NZ = g.nz + 1;
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int iu = blockIdx.x * 64 + threadIdx.x;
+    const int ju = blockIdx.y * 4 + threadIdx.y;
     const int k0 = blockIdx.z * kchunk, k1 = min(k0 + kchunk, NZ);
-    if (i >= NX || j >= NY) return;
+    // threads past the box edge compute a clamped node and store nothing
+    // (no early return: the PAP reduction needs every thread)
+    const bool valid = iu < NX && ju < NY;
+    const int i = min(iu, NX - 1), j = min(ju, NY - 1);
     const double* S = Ae + 64;
+    double xy = 0.0;
     const bool inxy = i >= 1 && i < g.nx && j >= 1 && j < g.ny;
     // the 9 loads of a plane go through a buffer descriptor of that plane:
     // a neighbour outside the box is an out-of-range offset and reads 0, so
@@ -92,8 +101,9 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
         const bool kst = k < k1;
         const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(
             y + (kst ? (int64_t)NX * NY * k : 0), (short)0, kst ? NX * NY * 8 : 0, 0x00020000);
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, acc), ry, (uint32_t)(i + NX * j) * 8u, 0,
-                                              0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, acc), ry,
+                                              valid ? (uint32_t)(i + NX * j) * 8u : 0x80000000u, 0, 0);
+        if (PAP && valid && kst) xy += b[4] * acc;
     };
     double wa[9], wb[9], wc[9], wd[9];
     load(k0 - 1, wa);
@@ -104,6 +114,98 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
         step(wb, wc, wd, wa, k + 1);
         step(wc, wd, wa, wb, k + 2);
         step(wd, wa, wb, wc, k + 3);
+    }
+    if (PAP) {
+        __shared__ double lds[4];
+        for (int off = 32; off > 0; off >>= 1) xy += __shfl_xor(xy, off, 64);
+        const int t = threadIdx.x + 64 * threadIdx.y;
+        if ((t & 63) == 0) lds[t >> 6] = xy;
+        __syncthreads();
+        const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        if (t == 0) {
+            pap[blk] = lds[0] + lds[1] + lds[2] + lds[3];
+            if (blk == 0) st->k += 1;
+        }
+    }
+}
+
+// The fused filter PCG (Jacobi, residual checked every iteration) in three
+// launches per iteration instead of six: the stencil with the p.Ap partials
+// (above), then
+//  * k_fupdate: every block sums the p.Ap partials (same order in every
+//    block), alpha = rz/pAp, x += alpha p, r -= alpha Ap, partials of r.r
+//    and r.(D^-1 r);
+//  * k_fcheck: every block sums those, rel = |r|/|b|; unless rel < tol,
+//    beta = rz_new/rz and p = D^-1 r + beta p; block 0 records the state.
+// The iteration count k is advanced by the stencil's block 0, so both
+// kernels read it unchanged; rz alternates between two slots by k's parity,
+// so block 0 can store the new value while the other blocks read the old.
+constexpr int FB = 256;   // blocks of the fused vector kernels
+__device__ __forceinline__ double bsum_bcast(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double t = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return t;
+}
+__device__ __forceinline__ double rz_old(const PcgState* st, int k) { return k == 1 ? st->rz : st->rzb[(k - 1) & 1]; }
+
+__global__ __launch_bounds__(256) void k_fupdate(int64_t n, PcgState* st, const double* __restrict__ pap, int npap,
+                                                 double* __restrict__ x, const double* __restrict__ p,
+                                                 double* __restrict__ r, const double* __restrict__ Ap,
+                                                 const double* __restrict__ dinv, double* __restrict__ part) {
+    if (st->halted) return;
+    __shared__ double lds[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npap; i += 256) s += pap[i];
+    const double pAp = bsum_bcast(s, lds);
+    const double a = rz_old(st, st->k) / pAp;
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)FB * 256) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        rr += ri * ri;
+        rz += ri * (ri * dinv[i]);
+    }
+    rr = bsum_bcast(rr, lds);
+    rz = bsum_bcast(rz, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = rr;
+        part[FB + blockIdx.x] = rz;
+        if (blockIdx.x == 0) {
+            st->pAp = pAp;
+            st->alpha = a;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fcheck(int64_t n, PcgState* st, const double* __restrict__ part,
+                                                double* __restrict__ p, const double* __restrict__ r,
+                                                const double* __restrict__ dinv) {
+    if (st->halted) return;
+    __shared__ double lds[4];
+    const double rr = bsum_bcast(part[threadIdx.x], lds);
+    const double rzn = bsum_bcast(part[FB + threadIdx.x], lds);
+    const int k = st->k;
+    const double rel = sqrt(rr) / st->bnorm;
+    const bool conv = rel < st->tol;
+    const double beta = rzn / rz_old(st, k);
+    if (!conv)
+        for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)FB * 256)
+            p[i] = dinv[i] * r[i] + beta * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rr = rr;
+        st->last_k = k;
+        st->rel = rel;
+        st->final_rel = rel;
+        if (conv) {
+            st->halted = PCG_CONVERGED;
+        } else {
+            st->beta = beta;
+            st->rzb[k & 1] = rzn;
+        }
     }
 }
 
@@ -145,14 +247,37 @@ __global__ void k_helm_extract(Grid3 g, const double* __restrict__ x, double sca
 
 }  // namespace
 
-void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const double* x, double* y) {
-    const dim3 block(64, 4, 1);
+// planes per thread: 16, fewer when that leaves under 2048 blocks, more when
+// it leaves over FUSED_PAP_MAX (the fused PCG's partials)
+void helm_shape(const Grid3& g, dim3* grid, int* kchunk) {
     const unsigned gx = (unsigned)((g.nx + 1 + 63) / 64), gy = (unsigned)((g.ny + 1 + 3) / 4);
     const int NZ = g.nz + 1;
-    int kchunk = 16;   // planes per thread; fewer when that leaves too few blocks
-    while (kchunk > 1 && (int64_t)gx * gy * ((NZ + kchunk - 1) / kchunk) < 2048) kchunk >>= 1;
-    const dim3 grid(gx, gy, (unsigned)((NZ + kchunk - 1) / kchunk));
-    k_helm_apply<<<grid, block, 0, s>>>(g, Ae, x, y, kchunk);
+    int kc = 16;
+    while (kc > 1 && (int64_t)gx * gy * ((NZ + kc - 1) / kc) < 2048) kc >>= 1;
+    while ((int64_t)gx * gy * ((NZ + kc - 1) / kc) > FUSED_PAP_MAX && kc < NZ) kc <<= 1;
+    *grid = dim3(gx, gy, (unsigned)((NZ + kc - 1) / kc));
+    *kchunk = kc;
+}
+
+void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const double* x, double* y) {
+    dim3 grid;
+    int kchunk;
+    helm_shape(g, &grid, &kchunk);
+    k_helm_apply<false><<<grid, dim3(64, 4, 1), 0, s>>>(g, Ae, x, y, kchunk, nullptr, nullptr);
+}
+
+int launch_filter_fused_iteration(hipStream_t s, const Grid3& g, const double* Ae, PcgState* st, double* x,
+                                  double* p, double* r, double* Ap, const double* dinv, double* partials) {
+    dim3 grid;
+    int kchunk;
+    helm_shape(g, &grid, &kchunk);
+    const int npap = (int)(grid.x * grid.y * grid.z);
+    if (npap > FUSED_PAP_MAX) return -1;
+    double* part2 = partials + FUSED_PAP_MAX;
+    k_helm_apply<true><<<grid, dim3(64, 4, 1), 0, s>>>(g, Ae, p, Ap, kchunk, st, partials);
+    k_fupdate<<<FB, 256, 0, s>>>(g.nn, st, partials, npap, x, p, r, Ap, dinv, part2);
+    k_fcheck<<<FB, 256, 0, s>>>(g.nn, st, part2, p, r, dinv);
+    return 0;
 }
 
 void helm_stencil(const double* Ae, double* S) {
