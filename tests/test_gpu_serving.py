@@ -113,3 +113,21 @@ def test_concurrent_captures_and_allocations():
             r, x = s.wait(s.submit(precision=(hexa.PREC_MIXED if i % 2 else hexa.PREC_FP64), **jobs[i]), want_x=True)
             assert r["status"] == 0
             assert np.array_equal(x, res[i][1]) and r["compliance"] == res[i][0]["compliance"]
+
+
+def test_device_memory_does_not_grow_with_oc_iterations():
+    """Every OC iteration rebuilds the multigrid hierarchy for the new
+    density; the rebuild reuses the handle's buffers, so a job's device
+    footprint is the same after 2 and after 6 iterations.  (Iteration 2 is
+    the first warm-started solve and adds the one n_dof guess vector; a
+    hierarchy per rebuild would add far more than that per iteration.)"""
+    say("memory per iteration")
+    job = dict(nx=32, ny=16, nz=16, support_face=0, n_loads=2, volfrac=0.3, seed=5)
+    n_dof = 3 * 33 * 17 * 17
+    with hexa.Server(workers=1) as s:
+        r1 = s.wait(s.submit(oc_iters=1, **job))
+        r2 = s.wait(s.submit(oc_iters=2, **job))
+        r6 = s.wait(s.submit(oc_iters=6, **job))
+    assert r1["status"] == 0 and r2["status"] == 0 and r6["status"] == 0
+    assert r2["device_bytes"] == r1["device_bytes"] + 8 * n_dof
+    assert r6["device_bytes"] == r2["device_bytes"]

@@ -154,7 +154,9 @@ int32_t mg_build(hexa_handle* h) {
     }
     const int L = choose_levels(h);
     const bool mx = mixed(h);
-    h->levels.assign(L, Level{});
+    // a rebuild (after every density change) reuses the levels' buffers: the
+    // level grids depend only on the handle's grid and options
+    if ((int)h->levels.size() != L) h->levels.assign(L, Level{});
     for (int l = 0; l < L; ++l) {
         Level& lv = h->levels[l];
         lv.g = l == 0 ? h->g
@@ -239,7 +241,41 @@ int32_t mg_set_shift(hexa_handle* h, double s) {
     Level& c = h->levels[L - 1];
     launch_fill<double>(h->stream, h->n_coarse * h->n_coarse, 0.0, h->d_Ainv);
     launch_coarse_assemble(h->stream, c.g, c.Ke64, c.fac64, c.mask, c.shift, h->d_Ainv);
-    launch_gauss_jordan(h->stream, h->n_coarse, h->d_Ainv, h->d_gj_row, h->d_gj_col);
+    // the sweep's 2 n launches replay from a graph captured once per handle
+    // and coarse size (launching them one by one costs more than running them)
+    const char* ge = getenv("HEXA_GRAPHS");
+    if (ge && ge[0] == '0') {
+        launch_gauss_jordan(h->stream, h->n_coarse, h->d_Ainv, h->d_gj_row, h->d_gj_col);
+    } else {
+        HX_TRY(check_launch());
+        if (!(h->gj_exec && h->gj_n == h->n_coarse && h->gj_A == h->d_Ainv)) {
+            CaptureGuard lock;
+            if (h->gj_exec) {
+                HX_CHECK(hipGraphExecDestroy(h->gj_exec));
+                h->gj_exec = nullptr;
+            }
+            HX_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            launch_gauss_jordan(h->stream, h->n_coarse, h->d_Ainv, h->d_gj_row, h->d_gj_col);
+            const hipError_t el = hipGetLastError();
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+            if (el != hipSuccess || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                set_error("coarse inverse capture failed: %s", hipGetErrorString(el != hipSuccess ? el : e));
+                return HEXA_E_DEVICE;
+            }
+            const hipError_t ei = hipGraphInstantiate(&h->gj_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) {
+                h->gj_exec = nullptr;
+                set_error("coarse inverse graph instantiation failed: %s", hipGetErrorString(ei));
+                return HEXA_E_DEVICE;
+            }
+            h->gj_n = h->n_coarse;
+            h->gj_A = h->d_Ainv;
+        }
+        HX_CHECK(hipGraphLaunch(h->gj_exec, h->stream));
+    }
     HX_TRY(check_launch());
     h->mg_shift = s;
     return HEXA_OK;
@@ -845,6 +881,7 @@ int32_t hexa_destroy(hexa_handle* h) {
     DeviceOpGuard lock;
     if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
     if (h->filt_exec) hipGraphExecDestroy(h->filt_exec);
+    if (h->gj_exec) hipGraphExecDestroy(h->gj_exec);
     for (void* p : h->allocs) hipFree(p);
     if (h->h_state) hipHostFree(h->h_state);
     for (int i = 0; i < RING; ++i)

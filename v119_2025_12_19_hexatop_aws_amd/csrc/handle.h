@@ -54,6 +54,11 @@ struct hexa_handle {
         int64_t gen = -1;
     } pcg_key;
     int64_t graph_gen = 0;
+    // the coarsest level's Gauss-Jordan sweep (two launches per pivot)
+    // captured as a graph, for n_coarse = gj_n on d_Ainv = gj_A
+    hipGraphExec_t gj_exec = nullptr;
+    int64_t gj_n = -1;
+    const double* gj_A = nullptr;
     hipGraphExec_t filt_exec = nullptr;   // one Helmholtz-filter PCG iteration
     int filt_every = 0;
     bool filt_fused = false;
