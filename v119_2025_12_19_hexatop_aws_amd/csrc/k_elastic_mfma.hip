@@ -150,15 +150,20 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const T* __restric
         }
     };
 
-    // gather items: (owned node o, component c), item = 3 o + c, items tid and tid + NT
+    // gather items: (owned node o, component c), item = NOWN c + o, items tid
+    // and tid + NT.  Component-major, so that neighbouring lanes read
+    // neighbouring elements of one Y row: with the node-major order three
+    // lanes in a row read one element of three rows NES apart, which share
+    // an LDS bank.
     int gq_n[2], gq_c[2];
     bool gq_in[2];
     int64_t gq_base[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int itm = tid + q * NT;
-        const int o = itm / 3;
-        gq_c[q] = itm - 3 * o;
+        const int c = itm / NOWN;
+        const int o = itm - NOWN * c;
+        gq_c[q] = c;
         gq_n[q] = o;
         const int gx = 1 + o % OWN, gy = 1 + o / OWN;
         gq_in[q] = itm < 3 * NOWN && i0 + gx < NX && j0 + gy < NY;
