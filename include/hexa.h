@@ -185,8 +185,8 @@ int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t prec
  * arrays, FP64): z = M^-1 r. */
 int32_t hexa_apply_precond(hexa_handle* h, const double* r, double* z);
 
-/* Seconds of GPU time of the last hexa_matvec_dev batch (HIP events on the
- * handle's stream), averaged per launch. */
+/* Milliseconds of GPU time of the last hexa_matvec_dev batch (HIP events on
+ * the handle's stream), averaged per launch. */
 int32_t hexa_last_kernel_ms(hexa_handle* h, double* ms);
 int32_t hexa_stream_sync(hexa_handle* h);
 
