@@ -15,8 +15,8 @@
 //    component each.  Both exchanges of a layer share one barrier: the node
 //    row staged is the one two planes ahead, and the buffers alternate with
 //    the layer parity.
-//  * z is marched, with the node planes and element factors of the next two
-//    layers in flight.  The butterfly is separable, so the x-y part of a
+//  * z is marched, with the node planes and element factors of the next three
+//    layers in flight (two in the first smoothing sweep).  The butterfly is separable, so the x-y part of a
 //    node plane's transform is computed once and used by the element layers
 //    below and above it, and the two layers' x-y modal results for a plane
 //    are summed before one inverse x-y butterfly: per element about 175
@@ -366,66 +366,87 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         if (dot_part && ot.has_out && owner) acc += (double)ulo[0] * o[0] + (double)ulo[1] * o[1] + (double)ulo[2] * o[2];
     };
 
-    // planes kb-1 and kb: own values, staged rows, modes of plane kb-1
-    T u0[3], u1[3], u2[3], up1[3], up2[3], PA[3][4], PB[3][4], YA[3][4], YB[3][4], xnx;
+    // Rings indexed by layer: with the chunk's layers counted from 0 at kb-1,
+    // layer j keeps the own values of planes kb-1+j.. in uu[j % 3..], its
+    // staged rows, x-y modes and modal results in the parity pair [j % 2],
+    // and the planes and element factors in flight in rr / ff[j % RING]:
+    // three layers ahead, or two in the first smoothing sweep, whose planes
+    // carry b and the inverse diagonal (three would cost it occupancy).
+    constexpr int RING = MODE == OP_SMOOTH0 ? 2 : 3;
+    T uu[3][3], up[2][3], PP[2][3][4], YY[2][3][4], xnx;
     {
         PIn p0, p1;
         load_in(kb - 1, p0);
         load_in(kb, p1);
-        own_value(p0, u0, xnx);
-        stage(P0{}, u0, xnx);
-        own_value(p1, u1, xnx);
-        stage(P1{}, u1, xnx);
+        own_value(p0, uu[0], xnx);
+        stage(P0{}, uu[0], xnx);
+        own_value(p1, uu[1], xnx);
+        stage(P1{}, uu[1], xnx);
         __syncthreads();
         T up0[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             up0[c] = su[0][w + 1][c][l];
-            up1[c] = su[1][w + 1][c][l];
+            up[0][c] = su[1][w + 1][c][l];
         }
-        plane_modes(u0, up0, PA);
+        plane_modes(uu[0], up0, PP[0]);
         __syncthreads();   // the staging buffers are rewritten by the first layer
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) YA[c][m] = T(0);
-    // two-deep prefetch ring: planes kl+2, kl+3 and factors of layers kl, kl+1
-    PIn r0, r1;
-    load_in(kb + 1, r0);
-    load_in(kb + 2, r1);
-    T f0 = load_fac(kb - 1), f1 = load_fac(kb);
-
-    // own values rotate u0 -> u1 -> u2 (planes kl, kl+1, kl+2), and the
-    // staged rows up1 -> up2, with the layer parity
-    for (int kl = kb - 1; kl < ke; kl += 2) {
-        {
-            Out ot;
-            load_out(kl, ot);
-            own_value(r0, u2, xnx);
-            const T fl = f0;
-            load_in(kl + 4, r0);
-            f0 = load_fac(kl + 2);
-            step(P0{}, PA, PB, YA, YB, u0, u1, up1, u2, xnx, up2, fl, ot);
-        }
-        if (kl + 1 >= ke) break;
-        {
-            Out ot;
-            load_out(kl + 1, ot);
-            own_value(r1, u0, xnx);
-            const T fl = f1;
-            load_in(kl + 5, r1);
-            f1 = load_fac(kl + 3);
-            step(P1{}, PB, PA, YB, YA, u1, u2, up2, u0, xnx, up1, fl, ot);
-        }
-        // planes kl+2, kl+3 are now in (u2, up2... u0) order: rotate names
+        for (int m = 0; m < 4; ++m) YY[0][c][m] = T(0);
+    // prefetch ring: planes kl+2 .. kl+1+RING and factors of layers kl .. kl-1+RING
+    PIn rr[RING];
+    T ff[RING];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T t = u1[c];
-            u1[c] = u0[c];
-            u0[c] = u2[c];
-            u2[c] = t;
+    for (int r = 0; r < RING; ++r) load_in(kb + 1 + r, rr[r]);
+#pragma unroll
+    for (int r = 0; r < RING; ++r) ff[r] = load_fac(kb - 1 + r);
+
+    // layer L = kb - 1 + j (+ a multiple of 6): consumes plane L+2 and the
+    // factor of layer L from ring slot j % RING and refills it with plane
+    // L+2+RING and the factor of layer L+RING
+    auto layer = [&](auto jc, const int L) {
+        constexpr int J = decltype(jc)::value;
+        constexpr int q = J % 2, s0 = J % 3, s1 = (J + 1) % 3, s2 = (J + 2) % 3, sr = J % RING;
+        Out ot;
+        load_out(L, ot);
+        own_value(rr[sr], uu[s2], xnx);
+        const T fl = ff[sr];
+        load_in(L + 2 + RING, rr[sr]);
+        ff[sr] = load_fac(L + RING);
+        step(std::integral_constant<int, q>{}, PP[q], PP[1 - q], YY[q], YY[1 - q], uu[s0], uu[s1], up[q], uu[s2],
+             xnx, up[1 - q], fl, ot);
+    };
+    if constexpr (RING == 2) {
+        // two layers per trip; the own-value ring is renamed by moves
+        // (unrolling to six layers costs this variant spills)
+        for (int kl = kb - 1; kl < ke; kl += 2) {
+            layer(std::integral_constant<int, 0>{}, kl);
+            if (kl + 1 >= ke) break;
+            layer(std::integral_constant<int, 1>{}, kl + 1);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T t = uu[1][c];
+                uu[1][c] = uu[0][c];
+                uu[0][c] = uu[2][c];
+                uu[2][c] = t;
+            }
         }
+    } else
+    for (int kl = kb - 1; kl < ke; kl += 6) {
+        layer(std::integral_constant<int, 0>{}, kl);
+        if (kl + 1 >= ke) break;
+        layer(std::integral_constant<int, 1>{}, kl + 1);
+        if (kl + 2 >= ke) break;
+        layer(std::integral_constant<int, 2>{}, kl + 2);
+        if (kl + 3 >= ke) break;
+        layer(std::integral_constant<int, 3>{}, kl + 3);
+        if (kl + 4 >= ke) break;
+        layer(std::integral_constant<int, 4>{}, kl + 4);
+        if (kl + 5 >= ke) break;
+        layer(std::integral_constant<int, 5>{}, kl + 5);
     }
     if (dot_part) {
         // fixed-order block sum: lanes by butterfly, then waves in order
