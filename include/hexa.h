@@ -352,7 +352,9 @@ typedef struct {
 int64_t hexa_estimate_bytes(const hexa_grid* grid, const hexa_options* opt);
 int32_t hexa_server_create(int32_t workers, int64_t mem_budget_bytes, hexa_server** out);
 int32_t hexa_server_submit(hexa_server* s, const hexa_job* job, int64_t* id);
-/* Blocks until the job ends; x_out (nE) receives the final design or NULL. */
+/* Blocks until the job ends; x_out (nE) receives the final design or NULL.
+ * One wait per job: the wait consumes the job's result, and a second wait
+ * on an id already waited for (or being waited for) returns HEXA_E_INVALID. */
 int32_t hexa_server_wait(hexa_server* s, int64_t id, hexa_job_result* res, double* x_out);
 double hexa_server_clock_ms(hexa_server* s);
 int32_t hexa_server_destroy(hexa_server* s);

@@ -279,16 +279,14 @@ void or_stiffness_factor(int64_t n, const double* rho, double E0, double Emin, d
 /* y = sum_e factor_e * Ke * x_e, element by element, with one full-length
  * buffer per task and a serial reduction in task order.
  * R/src/Solvers/CPUSolver.jl:56-122 (apply_stiffness).  Elements with
- * factor 0 are skipped (CPUSolver.jl:82).  nchunks <= 0 -> thread count. */
+ * factor 0 are skipped (CPUSolver.jl:82).  nchunks <= 0 -> 16 tasks (a
+ * fixed count, so the rounding does not depend on the machine's thread
+ * count; the reference uses one task per Julia thread). */
 void or_apply_stiffness(int nx, int ny, int nz, const double* Ke, const double* factor,
                         const double* x, double* y, int nchunks) {
     int64_t nE = (int64_t)nx * ny * nz;
     int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
-#ifdef _OPENMP
-    if (nchunks <= 0) nchunks = omp_get_max_threads();
-#else
-    if (nchunks <= 0) nchunks = 1;
-#endif
+    if (nchunks <= 0) nchunks = 16;
     int64_t chunk = (nE + nchunks - 1) / nchunks;
     double** parts = (double**)calloc((size_t)nchunks, sizeof(double*));
 #pragma omp parallel for schedule(static, 1)
@@ -346,10 +344,24 @@ void or_diag(int nx, int ny, int nz, const double* Ke, const double* factor, int
     }
 }
 
+/* Sums are formed over fixed blocks of SUM_BLOCK terms, block sums then
+ * added in block order: the result depends neither on the thread count nor
+ * on the order in which threads finish (an OpenMP reduction clause combines
+ * thread partials in completion order). */
+#define SUM_BLOCK 8192
 static double dotm(int64_t n, const double* a, const double* b) {
+    const int64_t nb = (n + SUM_BLOCK - 1) / SUM_BLOCK;
+    double* part = (double*)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nb; ++k) {
+        const int64_t i1 = (k + 1) * SUM_BLOCK < n ? (k + 1) * SUM_BLOCK : n;
+        double t = 0;
+        for (int64_t i = k * SUM_BLOCK; i < i1; ++i) t += a[i] * b[i];
+        part[k] = t;
+    }
     double s = 0;
-#pragma omp parallel for reduction(+:s) schedule(static)
-    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    for (int64_t k = 0; k < nb; ++k) s += part[k];
+    free(part);
     return s;
 }
 
@@ -976,8 +988,8 @@ int or_update_density(float* density, const float* l1, const uint8_t* prot, cons
 double or_simp_sensitivity(int nx, int ny, int nz, const double* Ke, const double* rho_phys,
                            double E0, double Emin, double penal, const double* U, double* dc) {
     int64_t nE = (int64_t)nx * ny * nz;
-    double c = 0.0;
-#pragma omp parallel for reduction(+:c) schedule(static)
+    double* ce_all = (double*)malloc(sizeof(double) * (size_t)(nE > 0 ? nE : 1));
+#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < nE; ++e) {
         int64_t nd[8];
         elem_nodes(e, nx, ny, nd);
@@ -990,9 +1002,14 @@ double or_simp_sensitivity(int nx, int ny, int nz, const double* Ke, const doubl
             ce += ue[r] * acc;
         }
         double rp = rho_phys[e];
-        c += (Emin + pow(rp, penal) * (E0 - Emin)) * ce;
+        ce_all[e] = (Emin + pow(rp, penal) * (E0 - Emin)) * ce;
         dc[e] = -penal * pow(rp, penal - 1.0) * (E0 - Emin) * ce;
     }
+    double* ones = (double*)malloc(sizeof(double) * (size_t)(nE > 0 ? nE : 1));
+    for (int64_t e = 0; e < nE; ++e) ones[e] = 1.0;
+    const double c = dotm(nE, ce_all, ones);
+    free(ones);
+    free(ce_all);
     return c;
 }
 

@@ -27,9 +27,17 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
 
+# Per-file flags.  The row-wave operator writes its fused multiply-adds out
+# and is compiled without automatic contraction: each copy of its unrolled
+# layer loop must round the same way, so that results do not depend on the
+# z-chunk length.
+FILE_FLAGS = {"k_elastic_rw.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src: str, build_dir: str = BUILD, extra=()) -> str:
     obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + FLAGS + list(extra)
+    cmd = ([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + FLAGS + FILE_FLAGS.get(src, [])
+           + list(extra))
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

@@ -446,7 +446,8 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
         const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask,
                                                h->d_p, nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart);
         if (nq > 0) {
-            launch_pcg_alpha_n(st, h->d_state, h->d_dotpart, nq, cpu ? 0 : 1);
+            launch_pcg_alpha_n(st, h->d_state, h->d_dotpart, nq, cpu ? 0 : 1,
+                               h->d_partials + (MAX_RED_Q - 1) * RED_BLOCKS);
         } else {
             launch_pcg_pAp(st, nd, h->d_state, h->d_p, h->d_Ap, h->d_partials);
             launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
@@ -727,7 +728,7 @@ int32_t ensure_solver_vectors(hexa_handle* h) {
     HX_TRY(dalloc(h, &h->d_b, nd));
     HX_TRY(dalloc(h, &h->d_diag, nd));
     HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
-    HX_TRY(dalloc(h, &h->d_dotpart, (size_t)std::max<int64_t>(1, elastic_rw_blocks(h->g))));
+    HX_TRY(dalloc(h, &h->d_dotpart, (size_t)std::max<int64_t>(1, elastic_rw_partials(h->g))));
     HX_TRY(dalloc(h, &h->d_scal, 16));
     HX_TRY(dalloc(h, &h->d_state, 1));
     return HEXA_OK;

@@ -131,8 +131,9 @@ void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke
 template <typename T, typename DT>
 void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                          const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
-// Returns the number of per-block (u, y) partial sums written to dot_part
-// (0 when dot_part is null or the kernel cannot provide them).
+// Returns the number of (u, y) partial sums written to dot_part (0 when
+// dot_part is null or the kernel cannot provide them); their values, and so
+// their fixed-order sum, do not depend on the kernel's z-chunk length.
 template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
@@ -141,8 +142,8 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
 // node plane of elem_bytes-sized values with 31-bit byte counts); otherwise
 // launch_elastic_rw falls back to the tile kernel, which has no OP_SMOOTH0.
 bool elastic_rw_supported(const Grid3& g, size_t elem_bytes);
-// Upper bound on the partial sums launch_elastic_rw writes for grid g.
-int64_t elastic_rw_blocks(const Grid3& g);
+// Number of partial sums launch_elastic_rw writes for grid g.
+int64_t elastic_rw_partials(const Grid3& g);
 // Picks the z-chunk length of the row-wave kernels for grid g by timing
 // (process-wide, once per grid and kernel); u is read, y overwritten.
 template <typename T>
@@ -260,7 +261,10 @@ void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const do
 void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials);
 void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
                          const double* r, const double* minv);
-void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int n, int breakdown_check);
+// alpha from the n (p, Ap) partials of the operator kernel, summed in a fixed
+// order (through RED_BLOCKS range sums in scratch when n > RED_BLOCKS).
+void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, int breakdown_check,
+                        double* scratch);
 void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
                               const double* Ap, double* partials);
 void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z,

@@ -27,7 +27,7 @@
 //    shared halo rows then come from its own L2.
 //
 // Every node sums its contributions in a fixed order: no atomics and the same
-// result on every run.
+// result on every run, whatever the z-chunk length.
 #include <algorithm>
 #include <array>
 #include <climits>
@@ -49,6 +49,7 @@ constexpr int OWN_X = WL - 2;    // owned node columns per tile
 constexpr int BY = HEXA_RW_BY;   // wavefronts per block = node rows per tile
 constexpr int OWN_Y = BY - 1;    // owned node rows per tile
 constexpr int NXCD = 8;
+constexpr int DOT_GROUP = 4;     // output planes per (u, y) partial sum
 
 // DPP wave shifts (GFX9 encodings): lane l reads lane l+1 / lane l-1; the
 // lane without a source reads 0.
@@ -128,6 +129,13 @@ __device__ __forceinline__ void ld3_dinv(Rsrc r, uint32_t off, const uint16_t*, 
     for (int c = 0; c < 3; ++c)
         o[c] = (T)__uint_as_float(((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off + 2 * c, 0, 0)) << 16);
 }
+
+// Fused multiply-add, written out: this file is compiled without automatic
+// contraction, so every copy of the unrolled layer loop performs the same
+// IEEE operations and a plane's result does not depend on which loop
+// position (and so which z-chunk length) produced it.
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // A wave-uniform coefficient held in a VGPR.  The FP64 kernel's scalar
 // registers cannot hold all of its coefficients; the coupling coefficients
@@ -323,19 +331,19 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
 #pragma unroll
                     for (int qq = 0; qq < 2; ++qq) {
                         const int mq = qq << d;
-                        yv[(1 << c) | mq] += cl[d][qq] * v[bb][(1 << bb) | mq];
-                        yv[(1 << bb) | mq] += cm[d][qq] * v[bb][(1 << c) | mq];
+                        yv[(1 << c) | mq] = fmad(cl[d][qq], v[bb][(1 << bb) | mq], yv[(1 << c) | mq]);
+                        yv[(1 << bb) | mq] = fmad(cm[d][qq], v[bb][(1 << c) | mq], yv[(1 << bb) | mq]);
                     }
                 }
                 // element factor and the z part of the transposed butterfly:
                 // plane kl gets (y_m - y_m+4), plane kl+1 gets (y_m + y_m+4)
                 T zlo[4];
                 Yout[c][0] = fl * yv[4];
-                zlo[0] = Yin[c][0] - fl * yv[4];
+                zlo[0] = fmad(-fl, yv[4], Yin[c][0]);
 #pragma unroll
                 for (int m = 1; m < 4; ++m) {
                     Yout[c][m] = fl * (yv[m] + yv[m + 4]);
-                    zlo[m] = Yin[c][m] + fl * (yv[m] - yv[m + 4]);
+                    zlo[m] = fmad(fl, yv[m] - yv[m + 4], Yin[c][m]);
                 }
                 // x-y part of the transposed butterfly -> corners cx + 2cy
                 const T t0 = zlo[0] - zlo[1], t1 = zlo[0] + zlo[1], t2 = zlo[2] - zlo[3], t3 = zlo[2] + zlo[3];
@@ -355,15 +363,16 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         for (int c = 0; c < 3; ++c) {
             upnx[c] = su[P][w + 1][c][l];
             const T below = sc[P][w > 0 ? w - 1 : 0][c][l];
-            const T Au = N0[c] + (owner ? below : T(0)) + shift * ulo[c];
+            const T Au = fmad(shift, ulo[c], N0[c] + (owner ? below : T(0)));
             T v;
             if (MODE == OP_APPLY) v = Au;
             else if (MODE == OP_RESID) v = ot.bv[c] - Au;
-            else v = ulo[c] + omega * ot.dv[c] * (ot.bv[c] - Au);   // OP_SMOOTH, OP_SMOOTH0
+            else v = fmad(omega * ot.dv[c], ot.bv[c] - Au, ulo[c]);   // OP_SMOOTH, OP_SMOOTH0
             o[c] = ((ot.mb >> c) & 1) ? T(0) : v;
         }
         st3(ot.ry, ooff, o);
-        if (dot_part && ot.has_out && owner) acc += (double)ulo[0] * o[0] + (double)ulo[1] * o[1] + (double)ulo[2] * o[2];
+        if (dot_part && ot.has_out && owner)
+            acc += fmad((double)ulo[2], (double)o[2], fmad((double)ulo[1], (double)o[1], (double)ulo[0] * o[0]));
     };
 
     // Rings indexed by layer: with the chunk's layers counted from 0 at kb-1,
@@ -404,6 +413,20 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
 #pragma unroll
     for (int r = 0; r < RING; ++r) ff[r] = load_fac(kb - 1 + r);
 
+    // (u, y) partial sums: each wave sums its lanes once per group of
+    // DOT_GROUP output planes and writes one value per (tile column, wave,
+    // group).  Chunks start on group boundaries, so every value, and the
+    // fixed-order sum over them, is the same for any chunk length.
+    const int tiles_xy = tiles_x * tiles_y;
+    const int64_t dot_base = (int64_t)(tx + tiles_x * ty) * BY + w;
+    auto flush_dot = [&](const int kl) {
+        if (kl >= kb && ((kl + 1) % DOT_GROUP == 0 || kl + 1 == ke)) {
+            double v = acc;
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (l == 0) dot_part[(int64_t)(kl / DOT_GROUP) * tiles_xy * BY + dot_base] = v;
+            acc = 0.0;
+        }
+    };
     // layer L = kb - 1 + j (+ a multiple of 6): consumes plane L+2 and the
     // factor of layer L from ring slot j % RING and refills it with plane
     // L+2+RING and the factor of layer L+RING
@@ -418,6 +441,7 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         ff[sr] = load_fac(L + RING);
         step(std::integral_constant<int, q>{}, PP[q], PP[1 - q], YY[q], YY[1 - q], uu[s0], uu[s1], up[q], uu[s2],
              xnx, up[1 - q], fl, ot);
+        if (dot_part) flush_dot(L);
     };
     if constexpr (RING == 2) {
         // two layers per trip; the own-value ring is renamed by moves
@@ -448,20 +472,6 @@ __global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef
         if (kl + 5 >= ke) break;
         layer(std::integral_constant<int, 5>{}, kl + 5);
     }
-    if (dot_part) {
-        // fixed-order block sum: lanes by butterfly, then waves in order
-        double v = acc;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        __syncthreads();
-        double* red = reinterpret_cast<double*>(&su[0][0][0]);
-        if (l == 0) red[w] = v;
-        __syncthreads();
-        if (l == 0 && w == 0) {
-            double tsum = 0.0;
-            for (int q = 0; q < BY; ++q) tsum += red[q];
-            dot_part[L] = tsum;
-        }
-    }
 }
 
 }  // namespace
@@ -483,12 +493,23 @@ int tuned_chunk(const Grid3& g, int kind) {
     return it == g_tuned.end() ? 0 : it->second;
 }
 
+// HEXA_RW_KCHUNK=n (1..64) fixes the chunk length of every launch that does
+// not ask for a specific one (used by the tests to cover every chunk length
+// and ring phase on small grids).
+int chunk_override() {
+    const char* e = getenv("HEXA_RW_KCHUNK");
+    if (!e || !e[0]) return 0;
+    const int v = atoi(e);
+    return v >= 1 && v <= 64 ? v : 0;
+}
+
 RwShape rw_shape(const Grid3& g, int kind = TK_RW64, int kchunk = 0) {
     RwShape r;
     r.tiles_x = (g.nx + 1 + OWN_X - 1) / OWN_X;
     r.tiles_y = (g.ny + 1 + OWN_Y - 1) / OWN_Y;
     // node planes per block: enough blocks to fill the chip, long enough
     // chunks to amortise the one recomputed element layer per block
+    if (!kchunk) kchunk = chunk_override();
     r.kchunk = kchunk ? kchunk : tuned_chunk(g, kind);
     if (!r.kchunk) {
         r.kchunk = 32;
@@ -504,10 +525,10 @@ bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
     return 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
 }
 
-// (chunks of one plane: the most blocks any choice launches)
-int64_t elastic_rw_blocks(const Grid3& g) {
-    const RwShape r = rw_shape(g, TK_RW64, 1);
-    return (int64_t)r.tiles_x * r.tiles_y * r.tiles_z;
+// One partial per (tile column, wave, group of DOT_GROUP output planes).
+int64_t elastic_rw_partials(const Grid3& g) {
+    const RwShape r = rw_shape(g, TK_RW64, DOT_GROUP);
+    return (int64_t)r.tiles_x * r.tiles_y * BY * ((g.nz + 1 + DOT_GROUP - 1) / DOT_GROUP);
 }
 
 template <typename T, typename DT>
@@ -519,7 +540,11 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
         launch_elastic_fact<T, DT>(s, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y);
         return 0;
     }
-    const RwShape r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32);
+    RwShape r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32);
+    // partial sums need chunks that start on group boundaries: multiples of
+    // DOT_GROUP planes (every candidate length is a power of two)
+    if (dot_part && r.kchunk % DOT_GROUP) r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32,
+                                                      ((r.kchunk + DOT_GROUP - 1) / DOT_GROUP) * DOT_GROUP);
     const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
     dim3 block(WL, BY);
     switch (mode) {
@@ -540,7 +565,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
                                                                   r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part);
             break;
     }
-    return dot_part ? tiles : 0;
+    return dot_part ? (int)elastic_rw_partials(g) : 0;
 }
 
 // Times K(rho)u (APPLY) for each candidate chunk length on this grid and

@@ -114,7 +114,21 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const doubl
     }
 }
 
-// alpha from n per-block partials of (p, Ap) written by the operator kernel.
+// First level of a long fixed-order sum: block b sums the b-th of RED_BLOCKS
+// equal contiguous ranges of the n partials.
+__global__ __launch_bounds__(RED_THREADS) void k_sum_ranges(const PcgState* st, const double* __restrict__ partials,
+                                                            int64_t n, double* __restrict__ out) {
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const int64_t len = (n + RED_BLOCKS - 1) / RED_BLOCKS;
+    const int64_t i0 = blockIdx.x * len, i1 = min(n, i0 + len);
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += RED_THREADS) s += partials[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// alpha from n partials of (p, Ap) written by the operator kernel.
 __global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const double* partials, int n,
                                                          int breakdown_check) {
     if (threadIdx.x == 0) st->x_done = st->x_iter;
@@ -360,8 +374,14 @@ void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, con
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check) {
     k_alpha<<<1, RED_THREADS, 0, s>>>(st, partials, breakdown_check);
 }
-void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int n, int breakdown_check) {
-    k_alpha_n<<<1, RED_THREADS, 0, s>>>(st, partials, n, breakdown_check);
+void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, int breakdown_check,
+                        double* scratch) {
+    if (n <= RED_BLOCKS) {
+        k_alpha_n<<<1, RED_THREADS, 0, s>>>(st, partials, (int)n, breakdown_check);
+    } else {
+        k_sum_ranges<<<RED_BLOCKS, RED_THREADS, 0, s>>>(st, partials, n, scratch);
+        k_alpha<<<1, RED_THREADS, 0, s>>>(st, scratch, breakdown_check);
+    }
 }
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
                        const double* Ap, const double* minv, double* partials, float* r32) {

@@ -43,6 +43,7 @@ struct Job {
     hexa_job_result res{};
     int64_t est = 0;
     bool done = false;
+    bool claimed = false;    // a hexa_server_wait is waiting for this job
     std::vector<double> x;   // final design
     std::string err;         // the worker's error message when status != OK
 };
@@ -282,8 +283,11 @@ int32_t hexa_server_wait(hexa_server* s, int64_t id, hexa_job_result* res, doubl
         auto it = s->jobs.find(id);
         if (it == s->jobs.end()) { set_error("unknown job id"); return HEXA_E_INVALID; }
         jb = it->second;
+        // one waiter per job: it alone erases and frees the entry
+        if (jb->claimed) { set_error("job %lld is already being waited for", (long long)id); return HEXA_E_INVALID; }
+        jb->claimed = true;
         s->cv_done.wait(lk, [&] { return jb->done; });
-        s->jobs.erase(it);
+        s->jobs.erase(id);
     }
     if (res) *res = jb->res;
     if (x_out) std::memcpy(x_out, jb->x.data(), sizeof(double) * jb->x.size());
