@@ -229,6 +229,24 @@ int32_t hexa_sensitivity(hexa_handle* h, double* dc, double* compliance);
 int32_t hexa_oc_update(hexa_handle* h, double* x, const double* dc, const double* dv, const double* w,
                        double volfrac, double move, double bisect_tol, double* lambda_out);
 
+/* hexa_oc_update_noalloc: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the OC update of hexa_oc_update on
+ * device arrays (d_x: nE doubles, in/out; d_dc, d_dv, d_w: nE doubles), with
+ * no handle and no device allocation.  Its scratch (the multisection's
+ * 32 x 1024 partial sums and the lambda bracket, 262,304 bytes) is a fixed
+ * array in device memory that every device the library is loaded on holds
+ * until the process ends; the per-element move-limit values are formed
+ * inside each pass, so the scratch does not grow with n (n in [1, 2^53]:
+ * the volume target is counted in FP64).  Same x and lambda as
+ * hexa_oc_update, bitwise.  Work is queued on stream (a hipStream_t, or NULL
+ * for the null stream); lambda_out (host) may be NULL, in which case the call
+ * does not synchronise (capture-safe); otherwise it waits for the stream.
+ * All calls on one device share its scratch: calls that overlap on one
+ * device race. */
+int32_t hexa_oc_update_noalloc(int64_t n, double* d_x, const double* d_dc, const double* d_dv, const double* d_w,
+                               double volfrac, double move, double bisect_tol, void* stream, double* lambda_out);
+
 typedef struct {
     double volfrac, move, rmin;   /* rmin = Helmholtz radius of the density filter */
     double filter_tol, bisect_tol;

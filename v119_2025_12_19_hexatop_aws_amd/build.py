@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhexa.so")
 
-SOURCES = ["capi.cpp", "capi_topopt.cpp", "server.cpp", "io.cpp", "ke.cpp", "k_elastic.hip", "k_elastic_fact.hip", "k_elastic_rw.hip", "k_elastic_mfma.hip", "k_mg.hip", "k_vector.hip", "k_pcg.hip", "k_filter.hip",
+SOURCES = ["capi.cpp", "capi_topopt.cpp", "capi_noalloc.hip", "server.cpp", "io.cpp", "ke.cpp", "k_elastic.hip", "k_elastic_fact.hip", "k_elastic_rw.hip", "k_elastic_mfma.hip", "k_mg.hip", "k_vector.hip", "k_pcg.hip", "k_filter.hip",
            "k_topopt.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

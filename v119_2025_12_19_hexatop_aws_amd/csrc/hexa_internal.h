@@ -224,6 +224,12 @@ int oc_bracket_bytes();
 void launch_oc(hipStream_t s, int64_t n, const double* dc, const double* dv, const double* w, double volfrac,
                double move, double tol, int max_passes, double* x, double* q, double* lo, double* hi, void* bracket,
                double* partials, double* out4);
+// The same update with q, lo and hi formed from x, dc and dv inside the
+// passes (no per-element work arrays; bitwise the same result as launch_oc):
+// scratch is the bracket (oc_bracket_bytes) and OC_NL * RED_BLOCKS partials.
+void launch_oc_direct(hipStream_t s, int64_t n, const double* dc, const double* dv, const double* w, double volfrac,
+                      double move, double tol, int max_passes, double* x, void* bracket, double* partials,
+                      double* out4);
 void launch_upd_proposed(hipStream_t s, int64_t n, const float* l1, const uint8_t* prot, const float* orig, float allow,
                          float E, float mind, double* prop);
 void launch_regrid(hipStream_t s, int onx, int ony, int onz, int nnx, int nny, int nnz, float ndx, float ndy,

@@ -170,20 +170,43 @@ struct OcBracket {
 };
 
 // q_e = x_e sqrt(max(-dc_e/dv_e, 0)); lo/hi = move-limit box in [0,1]
+__device__ __forceinline__ void oc_box(double x, double dc, double dv, double move, double& q, double& lo,
+                                       double& hi) {
+    const double be = -dc / dv;
+    q = x * sqrt(be > 0 ? be : 0.0);
+    lo = fmax(0.0, x - move);
+    hi = fmin(1.0, x + move);
+}
+
 __global__ void k_oc_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ dc,
                           const double* __restrict__ dv, double move, double* __restrict__ q,
                           double* __restrict__ lo, double* __restrict__ hi) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const double be = -dc[e] / dv[e];
-        q[e] = x[e] * sqrt(be > 0 ? be : 0.0);
-        lo[e] = fmax(0.0, x[e] - move);
-        hi[e] = fmin(1.0, x[e] + move);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        oc_box(x[e], dc[e], dv[e], move, q[e], lo[e], hi[e]);
+}
+
+// Element e's (q, lo, hi): read from the arrays k_oc_prep wrote (DIRECT =
+// false: a, b, c = q, lo, hi), or formed from the design, dc and dv on the
+// fly (DIRECT = true: a, b, c = x, dc, dv; no per-element work arrays).  Both
+// give the same values.
+template <bool DIRECT>
+__device__ __forceinline__ void oc_elem(const double* __restrict__ a, const double* __restrict__ b,
+                                        const double* __restrict__ c, double move, int64_t e, double& q, double& lo,
+                                        double& hi) {
+    if (DIRECT) {
+        oc_box(a[e], b[e], c[e], move, q, lo, hi);
+    } else {
+        q = a[e];
+        lo = b[e];
+        hi = c[e];
     }
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_oc_eval(int64_t n, const OcBracket* br, const double* __restrict__ q,
-                                                         const double* __restrict__ lo, const double* __restrict__ hi,
-                                                         const double* __restrict__ w, double* __restrict__ partials) {
+template <bool DIRECT>
+__global__ __launch_bounds__(RED_THREADS) void k_oc_eval(int64_t n, const OcBracket* br, const double* __restrict__ a,
+                                                         const double* __restrict__ b, const double* __restrict__ c,
+                                                         double move, const double* __restrict__ w,
+                                                         double* __restrict__ partials) {
     if (br->done) return;
     __shared__ double lds[RED_THREADS / 64];
     const double l1 = br->l1, l2 = br->l2;
@@ -194,9 +217,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_oc_eval(int64_t n, const OcBrac
 #pragma unroll
     for (int k = 0; k < OC_NL; ++k) inv[k] = 1.0 / sqrt(l1 + (l2 - l1) * (double)(k + 1) / (double)(OC_NL + 1));
     for (int64_t e = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; e < n; e += (int64_t)RED_BLOCKS * RED_THREADS) {
-        const double qe = q[e], le = lo[e], he = hi[e], we = w[e];
+        double qe, le, he;
+        oc_elem<DIRECT>(a, b, c, move, e, qe, le, he);
+        const double we = w[e];
 #pragma unroll
-        for (int k = 0; k < OC_NL; ++k) s[k] += we * fmin(fmax(qe * inv[k], le), he);
+        for (int k = 0; k < OC_NL; ++k) s[k] = fma(we, fmin(fmax(qe * inv[k], le), he), s[k]);
     }
 #pragma unroll
     for (int k = 0; k < OC_NL; ++k) {
@@ -229,19 +254,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_oc_pick(OcBracket* br, const do
     if ((b - a) / (a + b) <= br->tol || !(b > a)) br->done = 1;
 }
 
-// x_new at lambda = bracket midpoint; partial max |x_new - x|.
-__global__ __launch_bounds__(RED_THREADS) void k_oc_apply(int64_t n, const OcBracket* br, const double* __restrict__ q,
-                                                          const double* __restrict__ lo, const double* __restrict__ hi,
-                                                          const double* __restrict__ w, double* __restrict__ x,
+// x_new at lambda = bracket midpoint; partial max |x_new - x|.  (With
+// DIRECT, a is the design itself, read before it is overwritten.)
+template <bool DIRECT>
+__global__ __launch_bounds__(RED_THREADS) void k_oc_apply(int64_t n, const OcBracket* br, const double* a,
+                                                          const double* __restrict__ b, const double* __restrict__ c,
+                                                          double move, const double* __restrict__ w, double* x,
                                                           double* __restrict__ partials) {
     __shared__ double lds[RED_THREADS / 64];
     const double lm = 0.5 * (br->l1 + br->l2);
     const double inv = 1.0 / sqrt(lm);
     double ch = 0.0, vol = 0.0;
     for (int64_t e = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; e < n; e += (int64_t)RED_BLOCKS * RED_THREADS) {
-        const double xn = fmin(fmax(q[e] * inv, lo[e]), hi[e]);
+        double qe, le, he;
+        oc_elem<DIRECT>(a, b, c, move, e, qe, le, he);
+        const double xn = fmin(fmax(qe * inv, le), he);
         ch = fmax(ch, fabs(xn - x[e]));
-        vol += w[e] * xn;
+        vol = fma(w[e], xn, vol);
         x[e] = xn;
     }
     for (int off = 32; off > 0; off >>= 1) ch = fmax(ch, __shfl_xor(ch, off, 64));
@@ -501,10 +530,22 @@ void launch_oc(hipStream_t s, int64_t n, const double* dc, const double* dv, con
     k_oc_init<<<1, 1, 0, s>>>(br, volfrac * (double)n, tol);
     k_oc_prep<<<grid_for(n), 256, 0, s>>>(n, x, dc, dv, move, q, lo, hi);
     for (int p = 0; p < max_passes; ++p) {
-        k_oc_eval<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, br, q, lo, hi, w, partials);
+        k_oc_eval<false><<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, br, q, lo, hi, move, w, partials);
         k_oc_pick<<<1, RED_THREADS, 0, s>>>(br, partials);
     }
-    k_oc_apply<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, br, q, lo, hi, w, x, partials);
+    k_oc_apply<false><<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, br, q, lo, hi, move, w, x, partials);
+    k_oc_final<<<1, RED_THREADS, 0, s>>>(br, partials, out4);
+}
+void launch_oc_direct(hipStream_t s, int64_t n, const double* dc, const double* dv, const double* w, double volfrac,
+                      double move, double tol, int max_passes, double* x, void* bracket, double* partials,
+                      double* out4) {
+    OcBracket* br = (OcBracket*)bracket;
+    k_oc_init<<<1, 1, 0, s>>>(br, volfrac * (double)n, tol);
+    for (int p = 0; p < max_passes; ++p) {
+        k_oc_eval<true><<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, br, x, dc, dv, move, w, partials);
+        k_oc_pick<<<1, RED_THREADS, 0, s>>>(br, partials);
+    }
+    k_oc_apply<true><<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, br, x, dc, dv, move, w, x, partials);
     k_oc_final<<<1, RED_THREADS, 0, s>>>(br, partials, out4);
 }
 void launch_upd_proposed(hipStream_t s, int64_t n, const float* l1, const uint8_t* prot, const float* orig, float allow,

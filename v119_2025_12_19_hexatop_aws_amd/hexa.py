@@ -159,6 +159,8 @@ SIGNATURES = {
     "hexa_sensitivity": (C.c_int32, [_H, _P, C.POINTER(C.c_double)]),
     "hexa_oc_update": (C.c_int32, [_H, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
                                    C.POINTER(C.c_double)]),
+    "hexa_oc_update_noalloc": (C.c_int32, [C.c_int64, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double, _P,
+                                           C.POINTER(C.c_double)]),
     "hexa_oc_step": (C.c_int32, [_H, _P, C.POINTER(OcParams), C.POINTER(OcReport)]),
     "hexa_oc_step_dev": (C.c_int32, [_H, C.c_void_p, C.POINTER(OcParams), C.POINTER(OcReport)]),
     "hexa_update_density": (C.c_int32, [_H, _P, _P, _P, _P, C.POINTER(UpdateParams), C.POINTER(UpdateReport)]),
