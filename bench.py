@@ -49,10 +49,13 @@ def build_problem(cfg):
 
 
 def cpu_baseline(p, sample_matvecs=10):
-    """CPU oracle matvec (CPUSolver.apply_stiffness restated) on the host."""
+    """CPU oracle matvec (CPUSolver.apply_stiffness restated) on the host.
+    Threads: the host's core count, capped at 16, the host-core share one GPU
+    of the box is given (the rest belong to its other GPUs' jobs)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, 16)
     Ke = oracle.ke_hex8(1, 1, 1, 0.3)
     fac = oracle.stiffness_factor(p.rho, p.E0, p.Emin, p.penal)
     x = np.random.default_rng(0).standard_normal(p.n_dof)
@@ -62,10 +65,10 @@ def cpu_baseline(p, sample_matvecs=10):
         oracle.apply_stiffness(p.grid, Ke, fac, x, nchunks=threads)
     dt = (time.perf_counter() - t0) / sample_matvecs
     gbs = (8 * p.n_elem + 16 * p.n_dof) / dt / 1e9
-    return {"value": gbs, "unit": "GB/s", "cores": threads, "kind": "port",
+    return {"value": gbs, "unit": "GB/s", "cores": threads, "kind": "port", "host_nproc": nproc,
             "sample": f"{sample_matvecs} FP64 K(rho)u applications of the C oracle "
-                      f"(OpenMP, per-thread buffers) on the {p.nx}x{p.ny}x{p.nz} grid, "
-                      f"{dt * 1e3:.0f} ms each"}
+                      f"(OpenMP, {threads} tasks with per-task buffers) on the {p.nx}x{p.ny}x{p.nz} grid, "
+                      f"{dt * 1e3:.0f} ms each; {threads} of the host's {nproc} cores (one GPU's share)"}
 
 
 def serve(n_jobs, workers):
@@ -156,7 +159,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--jacobi-max-iter", type=int, default=3000)
-    ap.add_argument("--serve-jobs", type=int, default=40, help="jobs of the serving stream (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=400, help="jobs of the serving stream (0 = skip)")
     ap.add_argument("--serve-workers", type=int, default=8)
     ap.add_argument("--no-load-cases", action="store_true", help="skip the 8-load-case solve")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -176,8 +179,10 @@ def main():
     t = time.perf_counter()
     p = build_problem(args.config)
     log(f"problem {p.meta['name']}: nE={p.n_elem} nDOF={p.n_dof} ({time.perf_counter() - t:.1f}s)")
-    solver = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu,
-                         mg_levels=int(p.meta.get("mg_levels", 0)))
+    # measured ceilings of this GPU (vendor: 8 TB/s, 78.6 TFLOP/s FP64)
+    ceilings = hexa.device_ceilings(1 << 30)
+    log("ceilings", ceilings)
+    solver = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu)
     solver.set_problem(p.fixed, p.f[0])
     solver.set_density(p.rho)
 
@@ -227,28 +232,44 @@ def main():
     torch.cuda.empty_cache()
     shard = None if args.traffic_probe else shard_shape(p, args.steps, args.warmup)
 
-    solves = {}
+    solves, kernels = {}, {}
     if not args.no_solve:
-        u_ref = None
-        for name, prec, precond in (("mg_fp64", hexa.PREC_FP64, hexa.PRECOND_MG),
-                                    ("mg_mixed", hexa.PREC_MIXED, hexa.PRECOND_MG)):
-            s = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu, precision=prec,
-                            mg_levels=int(p.meta.get("mg_levels", 0)))
+        u_ref = {}
+        # automatic level count (coarsest grid <= 400 DOFs) and the configured
+        # one (6 levels for 512x256x256: coarsest 16x8x8)
+        lv_cfg = int(p.meta.get("mg_levels_config", 0))
+        runs = [("mg_fp64", hexa.PREC_FP64, 0), ("mg_mixed", hexa.PREC_MIXED, 0)]
+        if lv_cfg:
+            runs += [("mg_fp64_%dlev" % lv_cfg, hexa.PREC_FP64, lv_cfg), ("mg_mixed_%dlev" % lv_cfg, hexa.PREC_MIXED, lv_cfg)]
+        for name, prec, levels in runs:
+            s = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu, precision=prec, mg_levels=levels)
             s.set_problem(p.fixed, p.f[0])
             s.set_density(p.rho)
-            u, reps, st = s.solve(tol=1e-8, precond=precond, semantics=hexa.SEMANTICS_CPU, max_iter=2000)
+            u, reps, st = s.solve(tol=1e-8, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2000)
             r = reps[0]
             solves[name] = dict(status=st, iterations=r["iterations"], solve_s=r["solve_ms"] / 1e3,
                                 setup_s=r["setup_ms"] / 1e3, rel_res=r["rel_res"],
                                 true_rel_res=r["true_rel_res"], compliance=r["compliance"],
                                 mg_levels=r["mg_levels"])
             log(name, solves[name])
-            if name == "mg_fp64":
-                u_ref = u[0].copy()
+            ref = "mg_fp64" if levels == 0 else "mg_fp64_%dlev" % levels
+            if prec == hexa.PREC_FP64:
+                u_ref[levels] = u[0].copy()
             else:
-                solves[name]["u_rel_err_vs_fp64"] = float(np.linalg.norm(u[0] - u_ref) / np.linalg.norm(u_ref))
-                c64 = solves["mg_fp64"]["compliance"]
+                solves[name]["u_rel_err_vs_fp64"] = float(np.linalg.norm(u[0] - u_ref[levels])
+                                                          / np.linalg.norm(u_ref[levels]))
+                c64 = solves[ref]["compliance"]
                 solves[name]["compliance_rel_err_vs_fp64"] = abs(r["compliance"] - c64) / abs(c64)
+                solves[name]["speedup_vs_fp64"] = solves[ref]["solve_s"] / solves[name]["solve_s"]
+            if levels == 0:
+                # every kernel of the loop, timed on this handle's buffers
+                s.filter(p.rho, 1.5, tol=1e-3, max_iter=20, check_every=10)
+                ones = np.ones(p.n_elem)
+                s.oc_update(np.full(p.n_elem, 0.3), -ones, ones, ones, 0.3, 0.2, bisect_tol=1e-3)
+                kernels[name] = s.kernel_report(reps=10)
+                for k in kernels[name]:
+                    k["frac_hbm_peak"] = k["GBps"] / HBM_PEAK_GBS
+                log(name, "kernels", [(k["name"], round(k["ms"], 4), round(k["GBps"])) for k in kernels[name]])
             s.close()
             del u
         if not args.no_jacobi:
@@ -299,7 +320,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_elastic_rw<double>", "bytes_per_launch": nbytes64,
-                     "kernel_ms": ms_kernel},
+                     "kernel_ms": ms_kernel, "frac_of_measured_copy": achieved / ceilings["copy_GBps"]},
+        "ceilings": ceilings,
+        "kernels": kernels,
         "matvec_fp32": {"kernel_ms": ms32, "GBps": nbytes32 / (ms32 * 1e-3) / 1e9},
         "matvec_dense_widened_ke": dense,
         "solve": solves,

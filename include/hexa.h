@@ -185,6 +185,34 @@ int32_t hexa_matvec_dev(hexa_handle* h, const void* d_x, void* d_y, int32_t prec
  * arrays, FP64): z = M^-1 r. */
 int32_t hexa_apply_precond(hexa_handle* h, const double* r, double* z);
 
+/* Ceilings of this GPU, measured: STREAM-style copy and triad over arrays of
+ * 'bytes' each (GB/s counting every array read and written), and an FP64
+ * FMA loop (TFLOP/s, 2 flops per FMA).  Allocates and frees its own arrays.
+ * Any output may be NULL.  No reference counterpart. */
+int32_t hexa_device_ceilings(int64_t bytes, double* copy_gbps, double* triad_gbps, double* fma_tflops);
+
+/* Per-kernel time and achieved HBM rate of the solve loop's kernels on the
+ * handle's grid (no reference counterpart: the reference reports no kernel
+ * times).  Each kernel runs 'reps' times on the handle's stream between HIP
+ * events, on the handle's own buffers, after a solve with HEXA_PRECOND_MG
+ * (the V-cycle kernels of level 0 and the transfers to level 1 are timed at
+ * the handle's V-cycle precision).  The Helmholtz stencil is included when a
+ * filter has run on the handle, the OC multisection and SIMP sensitivity
+ * when an OC step or update has.  'bytes' is the kernel's algorithmic HBM
+ * traffic per launch: every array it must read or write, once.  The
+ * solution, density and load stay as they were; the solver's scratch
+ * vectors and the OC work arrays are overwritten.  Fills out[0..cap) and sets
+ * *count to the number of entries written. */
+typedef struct {
+    char name[40];
+    int32_t level;             /* multigrid level (0 = the handle's grid)     */
+    int32_t launches;          /* launches timed                              */
+    double ms;                 /* GPU time per launch                         */
+    double bytes;              /* algorithmic HBM bytes per launch            */
+    double gbps;               /* bytes / time, GB/s                          */
+} hexa_kernel_stat;
+int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, int32_t cap, int32_t* count);
+
 /* Milliseconds of GPU time of the last hexa_matvec_dev batch (HIP events on
  * the handle's stream), averaged per launch. */
 int32_t hexa_last_kernel_ms(hexa_handle* h, double* ms);

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhexa.so")
 
 SOURCES = ["capi.cpp", "capi_topopt.cpp", "capi_noalloc.hip", "server.cpp", "io.cpp", "ke.cpp", "k_elastic.hip", "k_elastic_fact.hip", "k_elastic_rw.hip", "k_elastic_mfma.hip", "k_mg.hip", "k_vector.hip", "k_pcg.hip", "k_filter.hip",
-           "k_topopt.hip"]
+           "k_topopt.hip", "k_probe.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

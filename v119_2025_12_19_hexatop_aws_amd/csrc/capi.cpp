@@ -1101,4 +1101,133 @@ int32_t hexa_stream_sync(hexa_handle* h) {
     return HEXA_OK;
 }
 
+int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, int32_t cap, int32_t* count) {
+    if (!h || !out || !count || cap < 1 || reps < 1) { set_error("invalid argument"); return HEXA_E_INVALID; }
+    *count = 0;
+    if (h->precond_ready != HEXA_PRECOND_MG || h->levels.size() < 2) {
+        set_error("hexa_kernel_report needs a multigrid solve (HEXA_PRECOND_MG, two levels or more) first");
+        return HEXA_E_INVALID;
+    }
+    hipStream_t st = h->stream;
+    const Grid3& g = h->g;
+    const int64_t nn = g.nn, ne = g.ne, nd = 3 * nn;
+    const bool mx = mixed(h);
+    // scratch vectors only: the solution (d_x), load (d_b) and diagonals stay
+    for (double* v : {h->d_p, h->d_Ap, h->d_z, h->d_best}) launch_fill<double>(st, nd, 0.5, v);
+    HX_TRY(check_launch());
+    int n = 0;
+    auto timed = [&](const char* name, int level, double bytes, auto&& launch) -> int32_t {
+        if (n >= cap) return HEXA_OK;
+        launch();   // warm
+        HX_CHECK(hipEventRecord(h->ev_a, st));
+        for (int r = 0; r < reps; ++r) launch();
+        HX_CHECK(hipEventRecord(h->ev_b, st));
+        HX_TRY(check_launch());
+        HX_CHECK(hipEventSynchronize(h->ev_b));
+        float ms = 0.f;
+        HX_CHECK(hipEventElapsedTime(&ms, h->ev_a, h->ev_b));
+        hexa_kernel_stat& o = out[n++];
+        std::memset(&o, 0, sizeof o);
+        std::snprintf(o.name, sizeof o.name, "%s", name);
+        o.level = level;
+        o.launches = reps;
+        o.ms = (double)ms / reps;
+        o.bytes = bytes;
+        o.gbps = bytes / (o.ms * 1e-3) / 1e9;
+        return HEXA_OK;
+    };
+    const double b64 = 8.0 * ne + 48.0 * nn;
+    HX_TRY(timed("K(rho)u fp64", 0, b64, [&] {
+        elastic<double, double>(h, g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, nullptr, h->d_p, nullptr, nullptr,
+                                0.0, 0.0, h->d_Ap);
+    }));
+    HX_TRY(timed("K(rho)u fp64 + (p,Ap) partials", 0, b64 + nn, [&] {
+        elastic<double, double>(h, g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask, h->d_p, nullptr, nullptr,
+                                0.0, 0.0, h->d_Ap, h->d_dotpart);
+    }));
+    // level 0 of the V-cycle and the transfers to and from level 1
+    Level& l0 = h->levels[0];
+    Level& l1 = h->levels[1];
+    const int L = (int)h->levels.size();
+    const double es = mx ? 4.0 : 8.0, ds = mx ? 2.0 : 8.0;   // vector and inverse-diagonal entry sizes
+    const double vec = 3.0 * es * nn, dinv = 3.0 * ds * nn, fac = es * ne, msk = (double)nn;
+    const double omega = h->opt.mg_omega;
+    if (L > 1) {
+        // (the FP64 V-cycle's level 0 reads and writes the caller's vectors:
+        // it is timed on the solver's scratch vectors)
+        auto sweep = [&](OpMode mode) {
+            if (mx)
+                elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, mode, l0.fac32, l0.mask, (const float*)l0.x,
+                                         (const float*)l0.b, (const uint16_t*)l0.dinv, (float)omega, (float)l0.shift,
+                                         (float*)l0.xt);
+            else
+                elastic<double, double>(h, g, l0.c64, l0.Ke64, mode, l0.fac64, l0.mask, h->d_best, h->d_z,
+                                        (const double*)l0.dinv, omega, l0.shift, (double*)l0.xt);
+        };
+        if (h->factored && elastic_rw_supported(g, mx ? 4 : 8))
+            HX_TRY(timed(mx ? "smooth from 0, fp32/bf16" : "smooth from 0, fp64", 0, dinv + vec * 2 + fac + msk,
+                         [&] { sweep(OP_SMOOTH0); }));
+        HX_TRY(timed(mx ? "smooth, fp32/bf16" : "smooth, fp64", 0, dinv + vec * 3 + fac + msk,
+                     [&] { sweep(OP_SMOOTH); }));
+        HX_TRY(timed(mx ? "residual, fp32" : "residual, fp64", 0, vec * 3 + fac + msk, [&] { sweep(OP_RESID); }));
+        const double cvec = 3.0 * es * l1.g.nn;
+        HX_TRY(timed(mx ? "restrict, fp32" : "restrict, fp64", 0, vec + cvec + l1.g.nn, [&] {
+            if (mx) launch_restrict<float>(st, g, l1.g, (const float*)l0.xt, l1.mask, (float*)l1.b);
+            else launch_restrict<double>(st, g, l1.g, (const double*)l0.xt, l1.mask, (double*)l1.b);
+        }));
+        HX_TRY(timed(mx ? "prolong + add, fp32" : "prolong + add, fp64", 0, 2 * vec + cvec + msk, [&] {
+            if (mx) launch_prolong_add<float>(st, g, l1.g, (const float*)l1.x, l0.mask, (float*)l0.xt);
+            else launch_prolong_add<double>(st, g, l1.g, (const double*)l1.x, l0.mask, (double*)l0.xt);
+        }));
+    }
+    // the PCG recurrence as the mixed (or FP64) multigrid solve runs it
+    launch_pcg_state_for_timing(st, h->d_state);
+    HX_TRY(check_launch());
+    float* r32 = mx ? (float*)l0.b : nullptr;
+    const float* z32 = mx ? (const float*)l0.x : nullptr;
+    if (mx) {
+        HX_TRY(timed("pcg update (r, r32)", 0, 28.0 * nd, [&] {
+            launch_pcg_update(st, nd, h->d_state, nullptr, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials, r32);
+        }));
+        HX_TRY(timed("pcg flexible dots (z32)", 0, 20.0 * nd, [&] {
+            launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_z, z32, h->d_Ap, h->d_partials);
+        }));
+        HX_TRY(timed("pcg p update + deferred x (z32)", 0, 36.0 * nd, [&] {
+            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, h->d_best);
+        }));
+    } else {
+        HX_TRY(timed("pcg update (x, r)", 0, 48.0 * nd, [&] {
+            launch_pcg_update(st, nd, h->d_state, h->d_best, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials);
+        }));
+        HX_TRY(timed("pcg flexible dots", 0, 24.0 * nd, [&] {
+            launch_pcg_flex_dots(st, nd, h->d_state, h->d_z, h->d_best, h->d_Ap, h->d_partials);
+        }));
+        HX_TRY(timed("pcg p update", 0, 24.0 * nd, [&] {
+            launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_best, nullptr, nullptr);
+        }));
+    }
+    if (h->d_Ae && h->d_hp && h->d_hAp)
+        HX_TRY(timed("Helmholtz stencil, fp64", 0, 16.0 * nn, [&] { launch_helm_apply(st, g, h->d_Ae, h->d_hp, h->d_hAp); }));
+    if (h->d_e0 && h->d_bracket && h->d_rho) {
+        double* e[7] = {h->d_e0, h->d_e1, h->d_e2, h->d_e3, h->d_e4, h->d_e5, h->d_e6};
+        launch_fill<double>(st, ne, 0.5, e[0]);
+        launch_fill<double>(st, ne, -1.0, e[1]);
+        launch_fill<double>(st, ne, 1.0, e[2]);
+        launch_fill<double>(st, ne, 1.0, e[3]);
+        HX_TRY(check_launch());
+        // one multisection pass reads q, lo, hi and w (OC_NL volumes at once)
+        HX_TRY(timed("OC multisection (16 passes)", 0, ne * (48.0 + 16 * 32.0 + 48.0), [&] {
+            launch_oc(st, ne, e[1], e[2], e[3], 0.3, 0.2, 1e-300, 16, e[0], e[4], e[5], e[6], h->d_bracket,
+                      h->d_partials, h->d_scal + 8);
+        }));
+        HX_TRY(timed("SIMP sensitivity", 0, 24.0 * nn + 16.0 * ne, [&] {
+            launch_simp_sens(st, g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, e[1],
+                             h->d_partials);
+        }));
+    }
+    HX_CHECK(hipStreamSynchronize(st));
+    *count = n;
+    return HEXA_OK;
+}
+
 }  // extern "C"

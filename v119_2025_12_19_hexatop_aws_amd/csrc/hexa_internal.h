@@ -251,6 +251,7 @@ void launch_upd_finish(hipStream_t s, int64_t n, const uint8_t* prot, const floa
 
 // PCG (k_pcg.hip).  All take the device state and do nothing once it halted.
 void launch_pcg_init_state(hipStream_t s, PcgState* st, const double* partials, double tol);
+void launch_pcg_state_for_timing(hipStream_t s, PcgState* st);
 void launch_pcg_rz(hipStream_t s, int64_t n, const double* r, const double* z, double* partials);
 void launch_pcg_set_rz(hipStream_t s, PcgState* st, const double* partials);
 void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, const double* Ap,

@@ -358,6 +358,17 @@ __global__ void k_best_copy(int64_t n, const PcgState* st, const double* __restr
 
 }  // namespace
 
+// A running state whose iteration has a deferred x update pending and
+// alpha = beta = 0, for timing the recurrence kernels in isolation.
+__global__ void k_state_for_timing(PcgState* st) {
+    st->halted = PCG_RUNNING;
+    st->k = 1;
+    st->x_iter = 1;
+    st->x_done = 0;
+    st->alpha = st->beta = 0.0;
+    st->rz = 1.0;
+}
+void launch_pcg_state_for_timing(hipStream_t s, PcgState* st) { k_state_for_timing<<<1, 1, 0, s>>>(st); }
 void launch_pcg_init_state(hipStream_t s, PcgState* st, const double* partials, double tol) {
     k_init_state<<<1, RED_THREADS, 0, s>>>(st, partials, tol);
 }

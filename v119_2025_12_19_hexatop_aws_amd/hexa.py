@@ -112,6 +112,15 @@ class UpdateReport(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("level", C.c_int32), ("launches", C.c_int32), ("ms", C.c_double),
+                ("bytes", C.c_double), ("gbps", C.c_double)]
+
+    def as_dict(self):
+        return dict(name=self.name.decode(), level=self.level, launches=self.launches, ms=self.ms,
+                    bytes=self.bytes, GBps=self.gbps)
+
+
 class Job(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("support_face", C.c_int32),
                 ("n_loads", C.c_int32), ("oc_iters", C.c_int32), ("precision", C.c_int32), ("pad", C.c_int32),
@@ -152,6 +161,9 @@ SIGNATURES = {
     "hexa_matvec_dev": (C.c_int32, [_H, _P, _P, C.c_int32, C.c_int32]),
     "hexa_apply_precond": (C.c_int32, [_H, _P, _P]),
     "hexa_last_kernel_ms": (C.c_int32, [_H, C.POINTER(C.c_double)]),
+    "hexa_device_ceilings": (C.c_int32, [C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
+    "hexa_kernel_report": (C.c_int32, [_H, C.c_int32, C.POINTER(KernelStat), C.c_int32, C.POINTER(C.c_int32)]),
     "hexa_stream_sync": (C.c_int32, [_H]),
     "hexa_filter": (C.c_int32, [_H, _P, _P, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                 C.POINTER(FilterReport)]),
@@ -220,6 +232,13 @@ def device_info():
     tot, ncu = C.c_int64(), C.c_int32()
     _check(lib().hexa_device_info(name, 256, C.byref(tot), C.byref(ncu)))
     return dict(name=name.value.decode(), total_bytes=tot.value, n_cu=ncu.value)
+
+
+def device_ceilings(nbytes=1 << 30):
+    """Measured copy / triad GB/s and FP64 FMA TFLOP/s of the current GPU."""
+    c, t, f = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().hexa_device_ceilings(int(nbytes), C.byref(c), C.byref(t), C.byref(f)))
+    return dict(copy_GBps=c.value, triad_GBps=t.value, fp64_fma_TFLOPs=f.value, array_bytes=int(nbytes))
 
 
 def default_options(**kw) -> Options:
@@ -321,6 +340,14 @@ class Solver:
         ms = C.c_double()
         _check(lib().hexa_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
+
+    def kernel_report(self, reps=10):
+        """Per-kernel GPU time and algorithmic HBM rate on this handle's grid
+        (after a multigrid solve); a list of dicts."""
+        out = (KernelStat * 32)()
+        n = C.c_int32()
+        _check(lib().hexa_kernel_report(self._h, int(reps), out, 32, C.byref(n)))
+        return [out[i].as_dict() for i in range(n.value)]
 
     def apply_precond(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1)
