@@ -235,6 +235,20 @@ typedef struct {
 int32_t hexa_filter(hexa_handle* h, const double* rho_in, double* rho_out, double radius, double tol,
                     int32_t max_iter, int32_t check_every, hexa_filter_report* rep);
 
+/* The reference's CPU filter fallback, run when its GPU filter is unavailable
+ * (TopOpt.jl:319-351): (R^2 K + M + 1e-9 I) x = b solved exactly (here: PCG
+ * to a relative residual of 1e-13) on the whole box when it has fewer than
+ * min_blocked elements (<= 0: the reference's 100000), else the domain
+ * decomposition of apply_domain_decomposed_cpu_filter! (TopOpt.jl:129-209):
+ * blocks of about target_block elements (<= 0: 50000; ceil(n / cbrt(target))
+ * blocks per axis), each solved alone on its box widened by
+ * max(2, ceil(3 radius / min h)) cells and clipped to the grid, the block
+ * cores written back.  That result approximates the global filter near the
+ * block faces.  rho_in/rho_out: nE host arrays; rep: largest block's
+ * iterations and residual. */
+int32_t hexa_filter_fallback(hexa_handle* h, const double* rho_in, double* rho_out, double radius,
+                             int64_t min_blocked, int32_t target_block, hexa_filter_report* rep);
+
 /* ---- element stress ------------------------------------------------------
  * Replaces Stress.compute_stress_field (Stress.jl:287-374): centroid stress
  * of every element with E_e = E * rho_e, von Mises, principal values

@@ -699,9 +699,21 @@ static void helm_apply(int nx, int ny, int nz, const double* Ae, const double* x
  * (rhs), :73-97 (matvec), :99-110 (extract), :112-123,153 (diagonal),
  * :190-266 (solve).  Returns iterations; *rel_out = last checked residual
  * (0 if none was checked, as the reference initialises it). */
+int or_filter_reg(int nx, int ny, int nz, double dx, double dy, double dz, double radius,
+                  int bug_compat, double reg, const double* rho, double* out, double tol, int max_iter,
+                  int check_every, double* rel_out);
 int or_filter(int nx, int ny, int nz, double dx, double dy, double dz, double radius,
               int bug_compat, const double* rho, double* out, double tol, int max_iter,
               int check_every, double* rel_out) {
+    return or_filter_reg(nx, ny, nz, dx, dy, dz, radius, bug_compat, 0.0, rho, out, tol, max_iter, check_every,
+                         rel_out);
+}
+
+/* The same with reg I added to the operator: the reference's CPU filter
+ * factorises R^2 K + M + 1e-9 I (R/src/Optimization/TopOpt.jl:72-77). */
+int or_filter_reg(int nx, int ny, int nz, double dx, double dy, double dz, double radius,
+                  int bug_compat, double reg, const double* rho, double* out, double tol, int max_iter,
+                  int check_every, double* rel_out) {
     int64_t nN = (int64_t)(nx + 1) * (ny + 1) * (nz + 1), nE = (int64_t)nx * ny * nz;
     double Ke8[64], Me8[64], Ae[64];
     or_helm_mats(dx, dy, dz, bug_compat, Ke8, Me8);
@@ -719,6 +731,7 @@ int or_filter(int nx, int ny, int nz, double dx, double dy, double dz, double ra
         elem_nodes(e, nx, ny, nd);
         for (int a = 0; a < 8; ++a) { b[nd[a]] += rho[e] * w; dg[nd[a]] += Ae[a * 8 + a]; }
     }
+    for (int64_t n = 0; n < nN; ++n) dg[n] += reg;
     double nb = sqrt(dotm(nN, b, b));
     int it = 0;
     double rel = 0.0;
@@ -731,6 +744,7 @@ int or_filter(int nx, int ny, int nz, double dx, double dy, double dz, double ra
     double rho_old = dotm(nN, r, z);
     for (it = 1; it <= max_iter; ++it) {
         helm_apply(nx, ny, nz, Ae, p, Ap);
+        for (int64_t n = 0; n < nN; ++n) Ap[n] += reg * p[n];
         double alpha = rho_old / dotm(nN, p, Ap);
         for (int64_t n = 0; n < nN; ++n) { x[n] += alpha * p[n]; r[n] -= alpha * Ap[n]; }
         if (check_every > 0 && it % check_every == 0) {

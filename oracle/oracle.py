@@ -73,6 +73,9 @@ def lib():
         L.or_filter.restype = C.c_int
         L.or_filter.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [
             C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.or_filter_reg.restype = C.c_int
+        L.or_filter_reg.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [
+            C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.or_stress.restype = None
         L.or_stress.argtypes = [C.c_int] * 3 + [C.c_double] * 5 + [_dp, _dp] + [C.c_void_p] * 6
         L.or_update_density.restype = C.c_int
@@ -206,6 +209,51 @@ def filter_density(grid, h, radius, rho, bug_compat=True, tol=1e-6, max_iter=200
     it = lib().or_filter(nx, ny, nz, h[0], h[1], h[2], radius, int(bug_compat), _f64(rho), out,
                          tol, max_iter, check_every, C.byref(rel))
     return out, it, rel.value
+
+
+def filter_density_reg(grid, h, radius, rho, reg, bug_compat=True, tol=1e-13, max_iter=20000, check_every=1):
+    """The filter with reg I added to the operator: the reference's CPU
+    filter, R^2 K + M + 1e-9 I (R/src/Optimization/TopOpt.jl:32-127)."""
+    nx, ny, nz = grid
+    out = np.zeros(nx * ny * nz)
+    rel = C.c_double()
+    it = lib().or_filter_reg(nx, ny, nz, h[0], h[1], h[2], radius, int(bug_compat), float(reg), _f64(rho), out,
+                             tol, max_iter, check_every, C.byref(rel))
+    return out, it, rel.value
+
+
+def filter_blocked(grid, h, radius, rho, bug_compat=True, target_block=50000, tol=1e-13):
+    """apply_domain_decomposed_cpu_filter! (R/src/Optimization/TopOpt.jl:129-209):
+    ceil(n / cbrt(target)) blocks per axis of ceil(n / chunks) elements, each
+    filtered alone (R^2 K + M + 1e-9 I, exact solve) on its box widened by
+    max(2, ceil(3R / min h)) cells and clipped to the grid; the block cores
+    are copied back."""
+    nx, ny, nz = grid
+    n = (nx, ny, nz)
+    rho = _f64(rho).reshape(nz, ny, nx)
+    cube = float(np.cbrt(target_block))
+    chunks = [max(1, int(np.ceil(v / cube))) for v in n]
+    blk = [-(-v // c) for v, c in zip(n, chunks)]
+    halo = max(2, int(np.ceil(3.0 * radius / min(h))))
+    out = np.zeros((nz, ny, nx))
+    for cz in range(chunks[2]):
+        for cy in range(chunks[1]):
+            for cx in range(chunks[0]):
+                c = (cx, cy, cz)
+                s0 = [c[a] * blk[a] for a in range(3)]
+                s1 = [min((c[a] + 1) * blk[a], n[a]) for a in range(3)]
+                if any(s0[a] >= s1[a] for a in range(3)):
+                    continue
+                h0 = [max(0, s0[a] - halo) for a in range(3)]
+                h1 = [min(n[a], s1[a] + halo) for a in range(3)]
+                loc = rho[h0[2]:h1[2], h0[1]:h1[1], h0[0]:h1[0]]
+                lg = (h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2])
+                res, _, _ = filter_density_reg(lg, h, radius, loc.ravel(), 1e-9, bug_compat, tol=tol)
+                res = res.reshape(lg[2], lg[1], lg[0])
+                out[s0[2]:s1[2], s0[1]:s1[1], s0[0]:s1[0]] = res[s0[2] - h0[2]:s1[2] - h0[2],
+                                                                  s0[1] - h0[1]:s1[1] - h0[1],
+                                                                  s0[0] - h0[0]:s1[0] - h0[0]]
+    return out.ravel()
 
 
 def stress(grid, h, E, nu, U, rho):

@@ -127,14 +127,17 @@ def cantilever_solve(nx, ny, nz, seed, penal=3.0, Emin=1e-9, shift_factor=1e-10)
     return rho, fixed, f, u, float(f @ u), s
 
 
-def filter_direct(nx, ny, nz, h, radius, rho, bug_compat):
+def filter_direct(nx, ny, nz, h, radius, rho, bug_compat, reg=0.0):
+    """(R^2 K + M + reg I) x = b by sparse LU; reg = 1e-9 is the reference's
+    CPU filter (TopOpt.jl:72-77), 0 its GPU filter."""
     K8, M8 = helm_exact(h, bug_compat)
     Ae = radius ** 2 * K8 + M8
     en = elem_nodes(nx, ny, nz)
     nN = (nx + 1) * (ny + 1) * (nz + 1)
     rows = np.repeat(en, 8, axis=1).ravel()
     cols = np.tile(en, (1, 8)).ravel()
-    A = sp.csr_matrix((np.tile(Ae.ravel(), len(en)), (rows, cols)), shape=(nN, nN)).tocsc()
+    A = sp.csr_matrix((np.tile(Ae.ravel(), len(en)), (rows, cols)), shape=(nN, nN))
+    A = (A + reg * sp.identity(nN)).tocsc()
     b = np.zeros(nN)
     np.add.at(b, en.ravel(), np.repeat(rho * h[0] * h[1] * h[2] / 8.0, 8))
     x = spla.spsolve(A, b)

@@ -347,3 +347,66 @@ def test_oracle_regrid_closed_forms():
     assert np.all(hard[soft > 0.4] == 1.0) and np.all(hard[soft < 0.2] == np.float32(1e-4))
     mid = (soft >= 0.2) & (soft <= 0.4)
     assert np.array_equal(hard[mid], soft[mid])
+
+
+# ---- the reference's CPU filter fallback (TopOpt.jl:32-209) -----------------
+def test_oracle_regularised_filter_matches_direct_solve():
+    """(R^2 K + M + 1e-9 I) x = b: the oracle's PCG to 1e-13 against sparse LU."""
+    from helpers import filter_direct
+    grid, h = (9, 6, 5), (1.0, 0.5, 2.0)
+    rho = np.random.default_rng(31).uniform(0.0, 1.0, 9 * 6 * 5)
+    out, it, rel = oracle.filter_density_reg(grid, h, 1.3, rho, 1e-9)
+    ref = filter_direct(*grid, h, 1.3, rho, True, reg=1e-9)
+    assert np.abs(out - ref).max() < 1e-11
+    # the shift is there: a large one moves the result
+    big, _, _ = oracle.filter_density_reg(grid, h, 1.3, rho, 1.0)
+    assert np.abs(big - filter_direct(*grid, h, 1.3, rho, True, reg=1.0)).max() < 1e-11
+    assert np.abs(big - ref).max() > 1e-3
+
+
+def _blocks(grid, target, radius, h):
+    n = grid
+    cube = float(np.cbrt(target))
+    chunks = [max(1, int(np.ceil(v / cube))) for v in n]
+    blk = [-(-v // c) for v, c in zip(n, chunks)]
+    halo = max(2, int(np.ceil(3.0 * radius / min(h))))
+    return chunks, blk, halo
+
+
+def test_oracle_blocked_filter_properties():
+    """apply_domain_decomposed_cpu_filter! restated: one block = the global
+    regularised solve; a constant field gives the constant closed form 2 rho/3
+    (reference element) in every block; a block's core depends on the density
+    inside its halo-widened box only; deep inside the cores the result is the
+    global filter's."""
+    from helpers import filter_direct
+    grid, h, R = (24, 14, 10), (1.0, 1.0, 1.0), 1.0
+    ne = 24 * 14 * 10
+    rho = np.random.default_rng(5).uniform(0.0, 1.0, ne)
+    glob = filter_direct(*grid, h, R, rho, True, reg=1e-9)
+    one = oracle.filter_blocked(grid, h, R, rho, target_block=24 ** 3)   # one block per axis
+    assert np.abs(one - glob).max() < 1e-11
+    cst = oracle.filter_blocked(grid, h, R, np.full(ne, 0.6), target_block=400)
+    assert np.abs(cst - 0.4).max() < 1e-9
+    chunks, blk, halo = _blocks(grid, 400, R, h)
+    assert chunks == [4, 2, 2] and blk == [6, 7, 5] and halo == 3
+    out = oracle.filter_blocked(grid, h, R, rho, target_block=400)
+    # block (1, 0, 1): core x 6..11, y 0..6, z 5..9; halo box x 3..14, y 0..9, z 2..9
+    r3 = rho.reshape(10, 14, 24).copy()
+    core = (slice(5, 10), slice(0, 7), slice(6, 12))
+    base = out.reshape(10, 14, 24)[core].copy()
+    r_out = r3.copy()
+    r_out[0:2, :, :] = 0.0          # z < 2: outside the halo box
+    r_out[:, :, 15:] = 0.0          # x > 14
+    assert np.array_equal(oracle.filter_blocked(grid, h, R, r_out.ravel(), target_block=400).reshape(10, 14, 24)[core],
+                          base)
+    r_in = r3.copy()
+    r_in[2, 3, 4] += 0.5            # inside the halo box, outside the core
+    moved = oracle.filter_blocked(grid, h, R, r_in.ravel(), target_block=400).reshape(10, 14, 24)[core]
+    assert np.abs(moved - base).max() > 1e-6
+    # at the centre of a core, three halo cells from every cut face, the
+    # blocked and global results agree closely
+    g3 = glob.reshape(10, 14, 24)
+    o3 = out.reshape(10, 14, 24)
+    assert abs(o3[7, 3, 8] - g3[7, 3, 8]) < 1e-3
+    assert np.abs(out - glob).max() > 1e-6       # but the decomposition is an approximation

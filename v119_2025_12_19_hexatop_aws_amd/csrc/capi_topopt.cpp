@@ -1,6 +1,8 @@
 // libhexa C ABI, optimisation step: Helmholtz filter, element stress, SIMP
 // sensitivity, optimality-criteria update, one full SIMP/OC iteration, and
 // the reference's stress-ratio density update.
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -18,11 +20,11 @@ namespace hexa {
 // residual checked every check_every iterations; d_out = element means of x.
 // R/src/Optimization/GPUHelmholtz.jl:190-266.
 int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double radius, double tol, int max_iter,
-                   int check_every, int* iters, double* rel, bool fused) {
+                   int check_every, int* iters, double* rel, bool fused, double reg) {
     hipStream_t st = h->stream;
     const Grid3& g = h->g;
     const int64_t nn = g.nn;
-    HX_TRY(dalloc(h, &h->d_Ae, 64 + 27));
+    HX_TRY(dalloc(h, &h->d_Ae, 64 + 27 + 1));
     HX_TRY(dalloc(h, &h->d_hb, (size_t)nn));
     HX_TRY(dalloc(h, &h->d_hx, (size_t)nn));
     HX_TRY(dalloc(h, &h->d_hr, (size_t)nn));
@@ -32,10 +34,11 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
     HX_TRY(dalloc(h, &h->d_scal, 16));
     HX_TRY(dalloc(h, &h->d_state, 1));
-    double K8[64], M8[64], Ae[64 + 27];
+    double K8[64], M8[64], Ae[64 + 27 + 1];
     helmholtz_element(h->grid.dx, h->grid.dy, h->grid.dz, h->opt.helm_element == HEXA_HELM_BUG_COMPAT, K8, M8);
     for (int i = 0; i < 64; ++i) Ae[i] = radius * radius * K8[i] + M8[i];
     helm_stencil(Ae, Ae + 64);
+    Ae[64 + 27] = reg;
     HX_CHECK(hipMemcpyAsync(h->d_Ae, Ae, sizeof Ae, hipMemcpyHostToDevice, st));
     const double w = h->grid.dx * h->grid.dy * h->grid.dz / 8.0;
     launch_helm_rhs_diag(st, g, h->d_Ae, d_in, w, h->d_hb, h->d_hdinv);
@@ -194,6 +197,101 @@ int32_t hexa_filter(hexa_handle* h, const double* rho_in, double* rho_out, doubl
         rep->iterations = it;
         rep->pad = 0;
         rep->rel_res = rel;
+        rep->ms = now_ms() - t0;
+    }
+    return HEXA_OK;
+}
+
+// The reference's CPU filter (TopOpt.jl:319-351): below min_blocked
+// elements the whole box is solved, (R^2 K + M + 1e-9 I) x = b with a direct
+// factorisation (:32-127); from min_blocked elements on, the box is split
+// into blocks of about target_block elements, each solved alone on its box
+// widened by max(2, ceil(3R / min h)) halo cells (clipped to the box), and
+// the block cores are copied back (:129-209).  The direct solves are PCG
+// solves here, to a relative residual of 1e-13.
+int32_t hexa_filter_fallback(hexa_handle* h, const double* rho_in, double* rho_out, double radius,
+                             int64_t min_blocked, int32_t target_block, hexa_filter_report* rep) {
+    if (!h || !rho_in || !rho_out) { set_error("null argument"); return HEXA_E_INVALID; }
+    if (!(radius >= 0)) { set_error("invalid filter radius"); return HEXA_E_INVALID; }
+    if (min_blocked <= 0) min_blocked = 100000;
+    if (target_block <= 0) target_block = 50000;
+    constexpr double REG = 1e-9, TOL = 1e-13;
+    constexpr int MAX_IT = 20000;
+    HX_TRY(ensure_elem(h));
+    const double t0 = now_ms();
+    hipStream_t st = h->stream;
+    const Grid3& g = h->g;
+    HX_CHECK(hipMemcpyAsync(h->d_e0, rho_in, sizeof(double) * g.ne, hipMemcpyHostToDevice, st));
+    int it_max = 0;
+    double rel_max = 0.0;
+    if (g.ne < min_blocked) {
+        HX_TRY(filter_dev(h, h->d_e0, h->d_e1, radius, TOL, MAX_IT, 1, &it_max, &rel_max, true, REG));
+    } else {
+        HX_CHECK(hipStreamSynchronize(st));
+        const double cube = std::cbrt((double)target_block);
+        const int n[3] = {g.nx, g.ny, g.nz};
+        int chunks[3], blk[3];
+        for (int a = 0; a < 3; ++a) {
+            chunks[a] = std::max(1, (int)std::ceil(n[a] / cube));
+            blk[a] = (n[a] + chunks[a] - 1) / chunks[a];
+        }
+        const double hmin = std::min(h->grid.dx, std::min(h->grid.dy, h->grid.dz));
+        const int halo = std::max(2, (int)std::ceil(3.0 * radius / hmin));
+        // one handle per distinct block box size, made on first use
+        std::vector<std::pair<std::array<int, 3>, hexa_handle*>> subs;
+        int32_t rc = HEXA_OK;
+        for (int cz = 0; cz < chunks[2] && rc == HEXA_OK; ++cz)
+            for (int cy = 0; cy < chunks[1] && rc == HEXA_OK; ++cy)
+                for (int cx = 0; cx < chunks[0] && rc == HEXA_OK; ++cx) {
+                    const int c[3] = {cx, cy, cz};
+                    int s0[3], s1[3], h0[3], h1[3];
+                    bool empty = false;
+                    for (int a = 0; a < 3; ++a) {
+                        s0[a] = c[a] * blk[a];                       // core [s0, s1), 0-based
+                        s1[a] = std::min((c[a] + 1) * blk[a], n[a]);
+                        empty = empty || s0[a] >= s1[a];
+                        h0[a] = std::max(0, s0[a] - halo);
+                        h1[a] = std::min(n[a], s1[a] + halo);
+                    }
+                    if (empty) continue;
+                    const std::array<int, 3> dims = {h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2]};
+                    hexa_handle* sh = nullptr;
+                    for (auto& kv : subs)
+                        if (kv.first == dims) sh = kv.second;
+                    if (!sh) {
+                        hexa_grid sg{dims[0], dims[1], dims[2], h->grid.dx, h->grid.dy, h->grid.dz};
+                        rc = hexa_create(&sg, &h->opt, &sh);
+                        if (rc != HEXA_OK) break;
+                        subs.emplace_back(dims, sh);
+                        rc = ensure_elem(sh);
+                        if (rc != HEXA_OK) break;
+                    }
+                    launch_box_gather(sh->stream, g, sh->g, h0[0], h0[1], h0[2], h->d_e0, sh->d_e0);
+                    int it = 0;
+                    double rel = 0.0;
+                    rc = filter_dev(sh, sh->d_e0, sh->d_e1, radius, TOL, MAX_IT, 1, &it, &rel, true, REG);
+                    if (rc != HEXA_OK) break;
+                    launch_box_scatter(sh->stream, g, sh->g, h0[0], h0[1], h0[2], s0[0] - h0[0], s0[1] - h0[1],
+                                       s0[2] - h0[2], s1[0] - s0[0], s1[1] - s0[1], s1[2] - s0[2], sh->d_e1, h->d_e1);
+                    rc = check_launch();
+                    it_max = std::max(it_max, it);
+                    rel_max = std::max(rel_max, rel);
+                }
+        for (auto& kv : subs) {
+            if (hipStreamSynchronize(kv.second->stream) != hipSuccess && rc == HEXA_OK) {
+                set_error("blocked filter: stream synchronisation failed");
+                rc = HEXA_E_DEVICE;
+            }
+            hexa_destroy(kv.second);
+        }
+        HX_TRY(rc);
+    }
+    HX_CHECK(hipMemcpyAsync(rho_out, h->d_e1, sizeof(double) * g.ne, hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    if (rep) {
+        rep->iterations = it_max;
+        rep->pad = 0;
+        rep->rel_res = rel_max;
         rep->ms = now_ms() - t0;
     }
     return HEXA_OK;

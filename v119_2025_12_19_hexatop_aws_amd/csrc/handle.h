@@ -146,7 +146,9 @@ int32_t poll_state(hexa_handle* h, int k, bool* halted);
 // Helmholtz filter of an element field on the handle's grid (device arrays).
 // fused: the three-launch iteration of launch_filter_fused_iteration
 // (residual checked every iteration; check_every must be 1).
+// reg: a diagonal shift reg I added to the filter operator (0 for the
+// reference's GPU filter, 1e-9 for its CPU one).
 int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double radius, double tol, int max_iter,
-                   int check_every, int* iters, double* rel, bool fused = false);
+                   int check_every, int* iters, double* rel, bool fused = false, double reg = 0.0);
 
 }  // namespace hexa

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <mutex>
 #include <shared_mutex>
@@ -64,6 +65,15 @@ constexpr uint8_t MASK_ALL = 0x7;
 // OP_SMOOTH0: one smoothing sweep started from x0 = omega D^-1 b, without
 // forming x0 (factored row-wave kernel only; u is not read).
 enum OpMode { OP_APPLY = 0, OP_RESID = 1, OP_SMOOTH = 2, OP_SMOOTH0 = 3 };
+
+// HEXA_FALLBACK_KERNELS=1 routes every elasticity product to the kernels that
+// otherwise run only on grids too large for the row-wave and matrix-core
+// kernels (node planes over 2 GB, 2^31 nodes or more): the factored tile
+// kernel and the node-centric dense kernel.  Used by the tests to cover them.
+inline bool fallback_kernels_forced() {
+    const char* e = getenv("HEXA_FALLBACK_KERNELS");
+    return e && e[0] == '1';
+}
 
 // Grid description of one level.
 struct Grid3 {
@@ -213,6 +223,11 @@ void helm_stencil(const double* Ae, double* S);
 void launch_helm_rhs_diag(hipStream_t s, const Grid3& g, const double* Ae, const double* rho, double w, double* b,
                           double* dinv);
 void launch_helm_extract(hipStream_t s, const Grid3& g, const double* x, double scale, double* out);
+// element copies between a grid and a sub-box of it (the blocked filter)
+void launch_box_gather(hipStream_t s, const Grid3& g, const Grid3& sub, int ox, int oy, int oz, const double* src,
+                       double* dst);
+void launch_box_scatter(hipStream_t s, const Grid3& g, const Grid3& sub, int ox, int oy, int oz, int cx, int cy,
+                        int cz, int sx, int sy, int sz, const double* src, double* dst);
 
 // Optimisation step (k_topopt.hip)
 void launch_stress(hipStream_t s, const Grid3& g, double dx, double dy, double dz, double E, double nu,

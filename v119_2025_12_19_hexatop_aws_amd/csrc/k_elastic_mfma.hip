@@ -277,7 +277,7 @@ __global__ __launch_bounds__(NT) void k_elastic_mfma(Grid3 g, const T* __restric
 }  // namespace
 
 bool elastic_mfma_supported(const Grid3& g) {
-    return (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) < (int64_t)1 << 31;
+    return !fallback_kernels_forced() && (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) < (int64_t)1 << 31;
 }
 
 template <typename T, typename DT>

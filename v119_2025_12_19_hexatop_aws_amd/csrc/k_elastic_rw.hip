@@ -522,7 +522,7 @@ RwShape rw_shape(const Grid3& g, int kind = TK_RW64, int kchunk = 0) {
 }  // namespace
 
 bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
-    return 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
+    return !fallback_kernels_forced() && 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
 }
 
 // One partial per (tile column, wave, group of DOT_GROUP output planes).

@@ -25,10 +25,11 @@ __device__ constexpr int FCY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 __device__ constexpr int FCZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 __device__ constexpr int fcorner(int cx, int cy, int cz) { return 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0)); }
 
-// y = A x, A = sum_e Ae over the elements of the box (scalar DOF per node).
-// Ae holds the 8x8 element matrix (64 values) followed by the 27-point stencil
-// of an interior node (27 values, offsets (dx,dy,dz) at dx+1 + 3(dy+1) +
-// 9(dz+1)): A has the same coefficients wherever all 8 elements around a node
+// y = A x, A = sum_e Ae over the elements of the box (scalar DOF per node)
+// plus reg I.  Ae holds the 8x8 element matrix (64 values), the 27-point
+// stencil of an interior node (27 values, offsets (dx,dy,dz) at dx+1 +
+// 3(dy+1) + 9(dz+1)) and reg (1 value; the reference's CPU filter adds
+// 1e-9 I, TopOpt.jl:75, its GPU filter nothing): A has the same coefficients wherever all 8 elements around a node
 // exist, so interior nodes take 27 FMAs; nodes on the box faces sum their
 // existing elements' rows.  A thread marches a chunk of planes along z with
 // the 3x3 neighbourhoods of planes k-1, k, k+1 in registers and those of
@@ -51,6 +52,7 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
     const bool valid = iu < NX && ju < NY;
     const int i = min(iu, NX - 1), j = min(ju, NY - 1);
     const double* S = Ae + 64;
+    const double reg = Ae[64 + 27];
     double xy = 0.0;
     const bool inxy = i >= 1 && i < g.nx && j >= 1 && j < g.ny;
     // the 9 loads of a plane go through a buffer descriptor of that plane:
@@ -98,6 +100,7 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
                 acc += ok ? r : 0.0;
             }
         }
+        acc = __builtin_fma(reg, b[4], acc);
         const bool kst = k < k1;
         const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(
             y + (kst ? (int64_t)NX * NY * k : 0), (short)0, kst ? NX * NY * 8 : 0, 0x00020000);
@@ -227,7 +230,28 @@ __global__ void k_helm_rhs_diag(Grid3 g, const double* __restrict__ Ae, const do
             d += Ae[l * 8 + l];
         }
         if (b) b[n] = s * w;
-        if (dinv) dinv[n] = 1.0 / d;
+        if (dinv) dinv[n] = 1.0 / (d + Ae[64 + 27]);
+    }
+}
+
+// Copies the elements of the box [o, o + sub) of grid g into the sub-grid's
+// element array (in), or the sub-grid elements of the box [co, co + csz)
+// (sub-grid coordinates) back to their place in g, shifted by o (out).
+__global__ void k_box_gather(Grid3 g, Grid3 sub, int ox, int oy, int oz, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < sub.ne;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % sub.nx), j = (int)((e / sub.nx) % sub.ny), k = (int)(e / ((int64_t)sub.nx * sub.ny));
+        dst[e] = src[(ox + i) + (int64_t)g.nx * ((oy + j) + (int64_t)g.ny * (oz + k))];
+    }
+}
+__global__ void k_box_scatter(Grid3 g, Grid3 sub, int ox, int oy, int oz, int cx, int cy, int cz, int sx, int sy,
+                              int sz, const double* __restrict__ src, double* __restrict__ dst) {
+    const int64_t n = (int64_t)sx * sy * sz;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = cx + (int)(e % sx), j = cy + (int)((e / sx) % sy), k = cz + (int)(e / ((int64_t)sx * sy));
+        dst[(ox + i) + (int64_t)g.nx * ((oy + j) + (int64_t)g.ny * (oz + k))] =
+            src[i + (int64_t)sub.nx * (j + (int64_t)sub.ny * k)];
     }
 }
 
@@ -303,6 +327,15 @@ void launch_helm_rhs_diag(hipStream_t s, const Grid3& g, const double* Ae, const
 }
 void launch_helm_extract(hipStream_t s, const Grid3& g, const double* x, double scale, double* out) {
     k_helm_extract<<<grid_for(g.ne), 256, 0, s>>>(g, x, scale, out);
+}
+void launch_box_gather(hipStream_t s, const Grid3& g, const Grid3& sub, int ox, int oy, int oz, const double* src,
+                       double* dst) {
+    k_box_gather<<<grid_for(sub.ne), 256, 0, s>>>(g, sub, ox, oy, oz, src, dst);
+}
+void launch_box_scatter(hipStream_t s, const Grid3& g, const Grid3& sub, int ox, int oy, int oz, int cx, int cy,
+                        int cz, int sx, int sy, int sz, const double* src, double* dst) {
+    k_box_scatter<<<grid_for((int64_t)sx * sy * sz), 256, 0, s>>>(g, sub, ox, oy, oz, cx, cy, cz, sx, sy, sz, src,
+                                                                   dst);
 }
 
 }  // namespace hexa

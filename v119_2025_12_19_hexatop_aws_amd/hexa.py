@@ -167,6 +167,7 @@ SIGNATURES = {
     "hexa_stream_sync": (C.c_int32, [_H]),
     "hexa_filter": (C.c_int32, [_H, _P, _P, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                 C.POINTER(FilterReport)]),
+    "hexa_filter_fallback": (C.c_int32, [_H, _P, _P, C.c_double, C.c_int64, C.c_int32, C.POINTER(FilterReport)]),
     "hexa_stress": (C.c_int32, [_H, _P, _P, C.c_double, _P, _P, _P, _P, _P, _P]),
     "hexa_sensitivity": (C.c_int32, [_H, _P, C.POINTER(C.c_double)]),
     "hexa_oc_update": (C.c_int32, [_H, _P, _P, _P, _P, C.c_double, C.c_double, C.c_double,
@@ -362,6 +363,17 @@ class Solver:
         rep = FilterReport()
         _check(lib().hexa_filter(self._h, _ptr(rho), _ptr(out), float(radius), float(tol), int(max_iter),
                                  int(check_every), C.byref(rep)))
+        return out, rep.iterations, rep.rel_res
+
+    def filter_fallback(self, rho, radius, min_blocked=0, target_block=0):
+        """The reference's CPU filter semantics (TopOpt.jl:319-351): global
+        regularised solve below min_blocked elements, halo-blocked domain
+        decomposition from there on; returns (filtered, iterations, rel)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(rho)
+        rep = FilterReport()
+        _check(lib().hexa_filter_fallback(self._h, _ptr(rho), _ptr(out), float(radius), int(min_blocked),
+                                          int(target_block), C.byref(rep)))
         return out, rep.iterations, rep.rel_res
 
     def stress(self, u=None, rho=None, E=1.0):
