@@ -63,6 +63,12 @@ enum { HEXA_PRECOND_JACOBI = 0, HEXA_PRECOND_MG = 1, HEXA_PRECOND_REF_MG = 2 };
 enum { HEXA_MATVEC_AUTO = 0, HEXA_MATVEC_DENSE = 1, HEXA_MATVEC_FACTORED = 2 };
 enum { HEXA_SEMANTICS_GPU = 0, HEXA_SEMANTICS_CPU = 1 };
 enum { HEXA_DIAG_EXACT = 0, HEXA_DIAG_X_ENTRY = 1 };
+/* Several load cases (nrhs > 1): HEXA_RHS_AUTO solves them together, each
+ * kernel of a PCG iteration working on all of them in one launch, when the
+ * solve is a V-cycle-preconditioned one with CPU semantics on the factored
+ * operator, and one after another otherwise; HEXA_RHS_SEQUENTIAL always one
+ * after another.  Both give each load case bitwise the same result. */
+enum { HEXA_RHS_AUTO = 0, HEXA_RHS_SEQUENTIAL = 1 };
 /* method reported by a solve; names follow GPUSolver.jl:314,463 */
 enum { HEXA_METHOD_JACOBI_GPU = 0, HEXA_METHOD_GMG_GPU = 1,
        HEXA_METHOD_FAILED_CORRUPT_INPUT = 2, HEXA_METHOD_JACOBI_CPU = 3,
@@ -102,7 +108,9 @@ typedef struct {
     int32_t semantics;         /* HEXA_SEMANTICS_*                          */
     int32_t warm_start;        /* use u_inout as the attempt-1 guess       */
     int32_t check_every;       /* 0 = semantics default (GPU 10, CPU 1)    */
-    int64_t reserved[4];
+    int32_t rhs_mode;          /* HEXA_RHS_*: how nrhs > 1 load cases run  */
+    int32_t pad;
+    int64_t reserved[3];
 } hexa_solve_params;
 
 typedef struct {
@@ -168,8 +176,10 @@ int32_t hexa_prune_voids(hexa_handle* h, double min_stiffness_threshold, int64_t
 int32_t hexa_solve(hexa_handle* h, double* u_inout, const hexa_solve_params* prm,
                    hexa_solve_report* rep);
 int32_t hexa_get_displacement(hexa_handle* h, double* u);
-/* Device pointer of the solution (nrhs*3*nN FP64), valid until the next
- * solve or destroy. */
+/* Device pointer of the solution (FP64), valid until the next solve or
+ * destroy.  Load cases solved together (HEXA_RHS_AUTO) are all kept, load
+ * case r at offset r*S with S = 3*nN rounded up to a multiple of 64; solved
+ * one after another, the last one is kept at offset 0. */
 int32_t hexa_displacement_dev(hexa_handle* h, const double** d_u);
 
 /* y = K(rho) x on all DOFs (no supports, no shift): the bare operator of

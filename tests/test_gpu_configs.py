@@ -104,7 +104,9 @@ def test_c4_eight_load_cases_at_workload():
         s.set_problem(p.fixed, p.f.ravel())
         s.set_density(p.rho)
         u64, reps64, st64 = s.solve(**kw)
-    assert st64 == hexa.OK
+        # FP64 load cases together and one after another: bitwise the same
+        u64s, reps64s, st64s = s.solve(rhs_mode=hexa.RHS_SEQUENTIAL, **kw)
+    assert st64 == st64s == hexa.OK and np.array_equal(u64, u64s)
     for r in range(8):
         c, c64 = reps[r]["compliance"], reps64[r]["compliance"]
         assert abs(c - c64) <= 1e-7 * abs(c64), r

@@ -363,6 +363,15 @@ def test_multi_rhs_load_cases(precision):
         assert rel(u[r], ud) < 1e-8
         assert abs(reps[r]["compliance"] - f[r] @ ud) <= 1e-8 * abs(f[r] @ ud)
     assert reps[1]["method_name"] == "None" and reps[1]["iterations"] == 0 and not np.any(u[1])
+    # the load cases solved together (the default here) and one after another
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, nrhs=nrhs) as s:
+        s.set_problem(p.fixed, f.ravel())
+        s.set_density(p.rho)
+        us, reps_s, sts = s.solve(rhs_mode=hexa.RHS_SEQUENTIAL, **kw)
+        ub, reps_b, stb = s.solve(**kw)       # again on the same handle (graph replay)
+    assert sts == stb == hexa.OK
+    assert np.array_equal(us, u) and np.array_equal(ub, u)
+    assert [r["iterations"] for r in reps_s] == [r["iterations"] for r in reps]
 
 
 @pytest.mark.parametrize("grid", [(8, 4, 4), (7, 5, 3)])

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -101,13 +102,18 @@ namespace {
 // f32-widened Ke).
 // With dot_part, the factored kernel also writes per-block partial sums of
 // (u, y); the return value is their count (0: not provided).
+// nb > 1: a batch of load cases, vectors batch_stride(3 nn) apart.
 template <typename T, typename DT>
 int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
             const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
-            double* dot_part = nullptr) {
+            double* dot_part = nullptr, int nb = 1) {
     if (h->factored)
-        return launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y, dot_part);
-    launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u, b, dinv, omega, shift, y);
+        return launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y, dot_part,
+                                        nb);
+    const int64_t vs = batch_stride(3 * g.nn);
+    for (int rb = 0; rb < nb; ++rb)
+        launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u ? u + rb * vs : u, b ? b + rb * vs : b, dinv,
+                                 omega, shift, y + rb * vs);
     return 0;
 }
 
@@ -199,13 +205,15 @@ int32_t mg_build(hexa_handle* h) {
         launch_diag<double>(h->stream, lv.g, lv.Ke64, lv.fac64, lv.mask, 0, lv.diag);
         const size_t es = mx ? sizeof(float) : sizeof(double);
         const bool coarsest = (l == L - 1);
+        // the level's work vectors hold a batch of load cases when nrhs > 1
+        const size_t nv = h->opt.nrhs > 1 ? (size_t)batch_stride(nd) * h->opt.nrhs : (size_t)nd;
         if (!coarsest) {
             HX_TRY(dalloc(h, &lv.dinv, (size_t)nd * (mx ? sizeof(uint16_t) : sizeof(double))));
-            HX_TRY(dalloc(h, &lv.xt, nd * es));
-            HX_TRY(dalloc(h, &lv.r, nd * es));
+            HX_TRY(dalloc(h, &lv.xt, nv * es));
+            HX_TRY(dalloc(h, &lv.r, nv * es));
         }
-        if (l > 0 || mx) HX_TRY(dalloc(h, &lv.b, nd * es));
-        if (l > 0 || mx) HX_TRY(dalloc(h, &lv.x, nd * es));
+        if (l > 0 || mx) HX_TRY(dalloc(h, &lv.b, nv * es));
+        if (l > 0 || mx) HX_TRY(dalloc(h, &lv.x, nv * es));
         HX_TRY(check_launch());
     }
     Level& c = h->levels[L - 1];
@@ -281,14 +289,16 @@ int32_t mg_set_shift(hexa_handle* h, double s) {
     return HEXA_OK;
 }
 
+// nb > 1: a batch of load cases (b, out and the level vectors hold nb
+// vectors, batch_stride(3 nn) apart on each level).
 template <typename T, typename DT>
-void vcycle_level(hexa_handle* h, int l, const T* b, T* out) {
+void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     Level& lv = h->levels[l];
     const int L = (int)h->levels.size();
     hipStream_t s = h->stream;
     const int64_t nd = 3 * lv.g.nn;
     if (l == L - 1) {
-        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out);
+        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out, nb);
         return;
     }
     const T* Ke = std::is_same<T, float>::value ? (const T*)lv.Ke32 : (const T*)lv.Ke64;
@@ -306,37 +316,39 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out) {
     const int swaps = (pre - first) + post;
     T* xa = (swaps % 2 == 0) ? out : (T*)lv.xt;
     T* xb = (swaps % 2 == 0) ? (T*)lv.xt : out;
-    if (fuse0) elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH0, fac, lv.mask, nullptr, b, dinv, omega, shift, xa);
-    else launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa);
+    if (fuse0) elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH0, fac, lv.mask, nullptr, b, dinv, omega, shift, xa, nullptr, nb);
+    else launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa, nb);
     for (int it = first; it < pre; ++it) {
-        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb);
+        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
         std::swap(xa, xb);
     }
-    elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r);
+    elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nullptr, nb);
     Level& nx = h->levels[l + 1];
-    launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b);
-    vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x);
-    launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa);
+    launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb);
+    vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x, nb);
+    launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb);
     for (int it = 0; it < post; ++it) {
-        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb);
+        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
         std::swap(xa, xb);
     }
 }
 
-// z = M^-1 r (FP64 in and out).
-void apply_mg(hexa_handle* h, const double* r, double* z) {
+// z = M^-1 r (FP64 in and out); nb load cases batch_stride(3 nn) apart.
+void apply_mg(hexa_handle* h, const double* r, double* z, int nb = 1) {
     if (h->levels.size() == 1) {
-        launch_dense_gemv<double>(h->stream, h->n_coarse, h->d_Ainv, r, z);
+        launch_dense_gemv<double>(h->stream, h->n_coarse, h->d_Ainv, r, z, nb);
         return;
     }
     if (mixed(h)) {
         Level& l0 = h->levels[0];
-        const int64_t nd = 3 * l0.g.nn;
-        launch_convert<double, float>(h->stream, nd, r, (float*)l0.b);
-        vcycle_level<float, uint16_t>(h, 0, (const float*)l0.b, (float*)l0.x);
-        launch_convert<float, double>(h->stream, nd, (const float*)l0.x, z);
+        const int64_t nd = 3 * l0.g.nn, vs = batch_stride(nd);
+        for (int rb = 0; rb < nb; ++rb)
+            launch_convert<double, float>(h->stream, nd, r + rb * vs, (float*)l0.b + rb * vs);
+        vcycle_level<float, uint16_t>(h, 0, (const float*)l0.b, (float*)l0.x, nb);
+        for (int rb = 0; rb < nb; ++rb)
+            launch_convert<float, double>(h->stream, nd, (const float*)l0.x + rb * vs, z + rb * vs);
     } else {
-        vcycle_level<double, double>(h, 0, r, z);
+        vcycle_level<double, double>(h, 0, r, z, nb);
     }
 }
 
@@ -712,25 +724,253 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     return success ? HEXA_OK : HEXA_E_NOT_CONVERGED;
 }
 
+// ------------------------------------------------------------ load cases
+// Several load cases solved together: the PCG of CPUSolver.jl:192-255 with
+// the V-cycle preconditioner, one recurrence per load case, advanced in lock
+// step with every kernel of an iteration working on all load cases in one
+// launch.  Each load case keeps its own state, scalars and partial sums in
+// the same order as a single solve, so its iterates are bitwise those of
+// solve_one.  An iteration launches as many kernels as a single solve's, each
+// over all load cases: the coarse levels, where a single solve's kernels
+// are too small to fill the GPU, gain the most.
+constexpr int32_t BATCH_UNSUPPORTED = 1;
+const int32_t k_converged_flag = PCG_CONVERGED;
+
+// Device states of the nb load cases of iteration k - LOOKAHEAD: all halted?
+int32_t poll_batch(hexa_handle* h, int k, int nb, bool* all_halted) {
+    const int slot = k % RING;
+    HX_CHECK(hipMemcpyAsync(&h->h_state[slot * nb], h->d_state, sizeof(PcgState) * nb, hipMemcpyDeviceToHost,
+                            h->stream));
+    HX_CHECK(hipEventRecord(h->ring_ev[slot], h->stream));
+    *all_halted = false;
+    if (k - LOOKAHEAD >= 1) {
+        const int kk = (k - LOOKAHEAD) % RING;
+        HX_CHECK(hipEventSynchronize(h->ring_ev[kk]));
+        bool all = true;
+        for (int r = 0; r < nb; ++r) all = all && h->h_state[kk * nb + r].halted != PCG_RUNNING;
+        *all_halted = all;
+    }
+    return HEXA_OK;
+}
+
+int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double s, int nb, const bool* active,
+                            AttemptResult* res) {
+    hipStream_t st = h->stream;
+    const int64_t nd = 3 * h->g.nn, vs = batch_stride(nd);
+    const int PS = BPART_STRIDE;
+    launch_make_dinv<double>(st, nd, h->d_diag, h->d_mask, s, 1, h->d_minv);
+    HX_TRY(mg_set_shift(h, s));
+    for (int rb = 0; rb < nb; ++rb) {
+        launch_dot2(st, nd, h->d_b + rb * vs, h->d_b + rb * vs, h->d_partials);
+        launch_pcg_init_state(st, h->d_state + rb, h->d_partials, prm->tol);
+    }
+    elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_RESID, h->d_fac, h->d_mask, h->d_x, h->d_b, nullptr,
+                            0.0, s, h->d_r, nullptr, nb);
+    apply_mg(h, h->d_r, h->d_z, nb);
+    HX_CHECK(hipMemcpyAsync(h->d_p, h->d_z, (size_t)(vs * nb) * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int rb = 0; rb < nb; ++rb) {
+        launch_pcg_rz(st, nd, h->d_r + rb * vs, h->d_z + rb * vs, h->d_partials);
+        launch_pcg_set_rz(st, h->d_state + rb, h->d_partials);
+        // a zero load on the free DOFs is solved by u = 0 (CPUSolver.jl:222-224)
+        if (!active[rb])
+            HX_CHECK(hipMemcpyAsync(&h->d_state[rb].halted, &k_converged_flag, sizeof(int32_t),
+                                    hipMemcpyHostToDevice, st));
+    }
+    HX_TRY(check_launch());
+    const bool mx = mixed(h) && h->levels.size() > 1;
+    float* r32 = mx ? (float*)h->levels[0].b : nullptr;
+    float* z32 = mx ? (float*)h->levels[0].x : nullptr;
+    const int64_t pstride = elastic_rw_partials(h->g);
+    auto iteration = [&]() -> int32_t {
+        const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask, h->d_p,
+                                               nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart, nb);
+        if (nq <= 0) {
+            set_error("batched load cases need the row-wave operator");
+            return HEXA_E_INVALID;
+        }
+        launch_pcg_alpha_n(st, h->d_state, h->d_dotpart, nq, 0, h->d_bscr, nb, pstride);
+        // (mixed: x += alpha p is deferred to the p update, which reads p anyway)
+        launch_pcg_update(st, nd, h->d_state, mx ? nullptr : h->d_x, h->d_p, h->d_r, h->d_Ap, nullptr, h->d_bpart,
+                          r32, nb, vs, PS);
+        launch_pcg_check(st, h->d_state, h->d_bpart, 1, 0, 1, 0, nb, PS);
+        if (mx) {
+            vcycle_level<float, uint16_t>(h, 0, r32, z32, nb);
+            launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_bpart, nb, vs, PS);
+            launch_pcg_flex_beta(st, h->d_state, h->d_bpart, nb, PS);
+            launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, h->d_x, nb, vs);
+        } else {
+            apply_mg(h, h->d_r, h->d_z, nb);
+            launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_bpart, nb, vs, PS);
+            launch_pcg_flex_beta(st, h->d_state, h->d_bpart, nb, PS);
+            launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr, nb, vs);
+        }
+        return check_launch();
+    };
+    const char* ge = getenv("HEXA_GRAPHS");
+    const bool graphs = !(ge && ge[0] == '0');
+    if (graphs && !(h->bpcg_exec && h->bpcg_key.nb == nb && h->bpcg_key.s == s && h->bpcg_key.gen == h->graph_gen)) {
+        CaptureGuard lock;
+        if (h->bpcg_exec) {
+            HX_CHECK(hipGraphExecDestroy(h->bpcg_exec));
+            h->bpcg_exec = nullptr;
+        }
+        HX_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int32_t rc = iteration();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc != HEXA_OK || e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (rc != HEXA_OK) return rc;
+            set_error("batched PCG iteration capture failed: %s", hipGetErrorString(e));
+            return HEXA_E_DEVICE;
+        }
+        const hipError_t ei = hipGraphInstantiate(&h->bpcg_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            h->bpcg_exec = nullptr;
+            set_error("batched PCG iteration graph instantiation failed: %s", hipGetErrorString(ei));
+            return HEXA_E_DEVICE;
+        }
+        h->bpcg_key.nb = nb;
+        h->bpcg_key.s = s;
+        h->bpcg_key.gen = h->graph_gen;
+    }
+    int k_done = 0;
+    bool stop = false;
+    for (int k = 1; k <= prm->max_iter && !stop; ++k) {
+        if (graphs) HX_CHECK(hipGraphLaunch(h->bpcg_exec, st));
+        else HX_TRY(iteration());
+        k_done = k;
+        HX_TRY(poll_batch(h, k, nb, &stop));
+    }
+    HX_CHECK(hipMemcpyAsync(h->h_state, h->d_state, sizeof(PcgState) * nb, hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    for (int rb = 0; rb < nb; ++rb) {
+        const PcgState& fs = h->h_state[rb];
+        res[rb].halted = fs.halted;
+        res[rb].iterations = fs.halted != PCG_RUNNING ? fs.last_k : k_done;
+        res[rb].final_rel = fs.final_rel;
+        res[rb].best_rel = fs.best_rel;
+    }
+    return HEXA_OK;
+}
+
+// All load cases of the handle with CPU semantics and the V-cycle, batched.
+// Returns BATCH_UNSUPPORTED (nothing solved) when the hierarchy cannot be
+// built, so that the caller solves them one by one (with the Jacobi fallback
+// of solve_one).
+int32_t solve_batched(hexa_handle* h, double* u_inout, const hexa_solve_params* prm, hexa_solve_report* rep) {
+    hipStream_t st = h->stream;
+    const int nb = h->opt.nrhs;
+    const int64_t nd = 3 * h->g.nn, vs = batch_stride(nd);
+    const double t0 = now_ms();
+    launch_diag<double>(st, h->g, h->d_Ke64, h->d_fac, h->d_mask, 0, h->d_diag);
+    launch_max_free(st, h->g, h->d_diag, h->d_mask, h->d_partials, h->d_scal);
+    for (int rb = 0; rb < nb; ++rb) {
+        k_mask_copy<<<grid_for(nd), 256, 0, st>>>(nd, h->d_f + rb * nd, h->d_mask, h->d_b + rb * vs);
+        launch_dot2(st, nd, h->d_b + rb * vs, h->d_b + rb * vs, h->d_partials);
+        launch_final_sum(st, 1, h->d_partials, h->d_scal + 16 + rb);
+    }
+    HX_TRY(check_launch());
+    std::vector<double> sc(16 + nb);
+    HX_CHECK(hipMemcpyAsync(sc.data(), h->d_scal, sizeof(double) * (16 + nb), hipMemcpyDeviceToHost, st));
+    HX_CHECK(hipStreamSynchronize(st));
+    const double maxd = sc[0];
+    if (std::isnan(maxd) || std::isinf(maxd)) return BATCH_UNSUPPORTED;   // solve_one reports it per case
+    if (h->precond_ready != HEXA_PRECOND_MG) {
+        const int32_t sb = mg_build(h);
+        if (sb == HEXA_E_OOM) return sb;
+        if (sb != HEXA_OK) return BATCH_UNSUPPORTED;
+        h->precond_ready = HEXA_PRECOND_MG;
+        h->mg_shift = -1.0;
+    }
+    const double s = prm->shift_factor * maxd;
+    HX_TRY(mg_set_shift(h, s));
+    tune_elastic_rw<double>(st, h->g, h->coef64, h->d_fac, h->d_x, h->d_best);
+    for (size_t l = 0; l + 1 < h->levels.size(); ++l) {
+        Level& lv = h->levels[l];
+        if (mixed(h)) tune_elastic_rw<float>(st, lv.g, lv.c32, lv.fac32, (const float*)lv.r, (float*)lv.xt);
+        else tune_elastic_rw<double>(st, lv.g, lv.c64, lv.fac64, (const double*)lv.r, (double*)lv.xt);
+    }
+    launch_fill<double>(st, vs * nb, 0.0, h->d_x);
+    HX_TRY(check_launch());
+    HX_CHECK(hipStreamSynchronize(st));
+    const double setup_ms = now_ms() - t0;
+    const double t1 = now_ms();
+    std::vector<AttemptResult> ar(nb);
+    std::unique_ptr<bool[]> active(new bool[nb]);
+    for (int rb = 0; rb < nb; ++rb) active[rb] = sc[16 + rb] != 0.0;
+    HX_TRY(run_attempt_batched(h, prm, s, nb, active.get(), ar.data()));
+    const double solve_ms = now_ms() - t1;
+    bool all_ok = true;
+    for (int rb = 0; rb < nb; ++rb) {
+        double* x = h->d_x + rb * vs;
+        double* b = h->d_b + rb * vs;
+        double* r = h->d_r + rb * vs;
+        launch_dot2(st, nd, b, x, h->d_partials);
+        launch_final_sum(st, 1, h->d_partials, h->d_scal);
+        elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_RESID, h->d_fac, h->d_mask, x, b, nullptr, 0.0, s,
+                                r);
+        launch_dot2(st, nd, r, r, h->d_partials);
+        launch_final_sum(st, 1, h->d_partials, h->d_scal + 1);
+        launch_dot2(st, nd, b, b, h->d_partials);
+        launch_final_sum(st, 1, h->d_partials, h->d_scal + 2);
+        HX_TRY(check_launch());
+        double v[3];
+        HX_CHECK(hipMemcpyAsync(v, h->d_scal, sizeof v, hipMemcpyDeviceToHost, st));
+        if (u_inout) HX_CHECK(hipMemcpyAsync(u_inout + rb * nd, x, nd * sizeof(double), hipMemcpyDeviceToHost, st));
+        HX_CHECK(hipStreamSynchronize(st));
+        hexa_solve_report R{};
+        const bool none = !active[rb];
+        const bool ok = none || ar[rb].halted == PCG_CONVERGED;
+        all_ok = all_ok && ok;
+        R.method = none ? HEXA_METHOD_NONE : HEXA_METHOD_JACOBI_CPU;
+        R.compliance = v[0];
+        R.true_rel_res = v[2] > 0 ? std::sqrt(v[1] / v[2]) : 0.0;
+        R.rel_res = none ? 0.0 : ar[rb].final_rel;
+        R.iterations = none ? 0 : ar[rb].iterations;
+        R.attempt = 1;
+        R.shift_used = s;
+        R.converged = ok ? 1 : 0;
+        R.matvecs = none ? 0 : ar[rb].iterations + 1;
+        R.mg_levels = (int32_t)h->levels.size();
+        // the batch's set-up and solve time, shared evenly by its load cases
+        R.setup_ms = setup_ms / nb;
+        R.solve_ms = solve_ms / nb;
+        if (rep) rep[rb] = R;
+    }
+    h->have_solution = true;
+    h->last_mg = 1;
+    return all_ok ? HEXA_OK : HEXA_E_NOT_CONVERGED;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
 namespace hexa {
 int32_t ensure_solver_vectors(hexa_handle* h) {
     const size_t nd = (size_t)(3 * h->g.nn);
-    HX_TRY(dalloc(h, &h->d_x, nd));
-    HX_TRY(dalloc(h, &h->d_r, nd));
-    HX_TRY(dalloc(h, &h->d_p, nd));
-    HX_TRY(dalloc(h, &h->d_Ap, nd));
-    HX_TRY(dalloc(h, &h->d_z, nd));
+    // with several load cases the iterate vectors hold a batch of them,
+    // batch_stride(nd) entries apart (the batched PCG)
+    const int nb = h->opt.nrhs;
+    const size_t nv = nb > 1 ? (size_t)batch_stride((int64_t)nd) * nb : nd;
+    HX_TRY(dalloc(h, &h->d_x, nv));
+    HX_TRY(dalloc(h, &h->d_r, nv));
+    HX_TRY(dalloc(h, &h->d_p, nv));
+    HX_TRY(dalloc(h, &h->d_Ap, nv));
+    HX_TRY(dalloc(h, &h->d_z, nv));
     HX_TRY(dalloc(h, &h->d_minv, nd));
     HX_TRY(dalloc(h, &h->d_best, nd));
-    HX_TRY(dalloc(h, &h->d_b, nd));
+    HX_TRY(dalloc(h, &h->d_b, nv));
     HX_TRY(dalloc(h, &h->d_diag, nd));
     HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
-    HX_TRY(dalloc(h, &h->d_dotpart, (size_t)std::max<int64_t>(1, elastic_rw_partials(h->g))));
-    HX_TRY(dalloc(h, &h->d_scal, 16));
-    HX_TRY(dalloc(h, &h->d_state, 1));
+    HX_TRY(dalloc(h, &h->d_dotpart, (size_t)std::max<int64_t>(1, elastic_rw_partials(h->g)) * nb));
+    HX_TRY(dalloc(h, &h->d_scal, 16 + (size_t)nb));
+    HX_TRY(dalloc(h, &h->d_state, (size_t)nb));
+    if (nb > 1) {
+        HX_TRY(dalloc(h, &h->d_bpart, (size_t)BPART_STRIDE * nb));
+        HX_TRY(dalloc(h, &h->d_bscr, (size_t)RED_BLOCKS * nb));
+    }
     return HEXA_OK;
 }
 
@@ -857,7 +1097,7 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
         DeviceOpGuard lock;
         ok = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess &&
              hipEventCreate(&h->ev_a) == hipSuccess && hipEventCreate(&h->ev_b) == hipSuccess &&
-             hipHostMalloc((void**)&h->h_state, sizeof(PcgState) * RING, hipHostMallocDefault) == hipSuccess;
+             hipHostMalloc((void**)&h->h_state, sizeof(PcgState) * RING * o.nrhs, hipHostMallocDefault) == hipSuccess;
         for (int i = 0; ok && i < RING; ++i)
             ok = hipEventCreateWithFlags(&h->ring_ev[i], hipEventDisableTiming) == hipSuccess;
     }
@@ -881,6 +1121,7 @@ int32_t hexa_destroy(hexa_handle* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     DeviceOpGuard lock;
     if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
+    if (h->bpcg_exec) hipGraphExecDestroy(h->bpcg_exec);
     if (h->filt_exec) hipGraphExecDestroy(h->filt_exec);
     if (h->gj_exec) hipGraphExecDestroy(h->gj_exec);
     for (void* p : h->allocs) hipFree(p);
@@ -1002,6 +1243,11 @@ int32_t hexa_solve(hexa_handle* h, double* u_inout, const hexa_solve_params* prm
     }
     HX_TRY(ensure_solver_vectors(h));
     const int64_t nd = 3 * h->g.nn;
+    if (h->opt.nrhs > 1 && p.rhs_mode != HEXA_RHS_SEQUENTIAL && p.semantics == HEXA_SEMANTICS_CPU &&
+        p.precond == HEXA_PRECOND_MG && h->factored && elastic_rw_supported(h->g, sizeof(double))) {
+        const int32_t sb = solve_batched(h, u_inout, &p, rep);
+        if (sb != BATCH_UNSUPPORTED) return sb;
+    }
     int32_t worst = HEXA_OK;
     for (int r = 0; r < h->opt.nrhs; ++r) {
         int32_t s = solve_one(h, r, u_inout ? u_inout + r * nd : nullptr, &p, rep ? rep + r : nullptr);
@@ -1145,6 +1391,17 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
         elastic<double, double>(h, g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask, h->d_p, nullptr, nullptr,
                                 0.0, 0.0, h->d_Ap, h->d_dotpart);
     }));
+    const int nb = h->opt.nrhs;
+    if (nb > 1 && h->factored && elastic_rw_supported(g, sizeof(double))) {
+        // the load cases of a batch in one launch: element factors and flags
+        // are algorithmic bytes once
+        char nm[40];
+        std::snprintf(nm, sizeof nm, "K(rho)u fp64, %d load cases at once", nb);
+        HX_TRY(timed(nm, 0, 8.0 * ne + 48.0 * nn * nb, [&] {
+            elastic<double, double>(h, g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, nullptr, h->d_p, nullptr, nullptr,
+                                    0.0, 0.0, h->d_Ap, nullptr, nb);
+        }));
+    }
     // level 0 of the V-cycle and the transfers to and from level 1
     Level& l0 = h->levels[0];
     Level& l1 = h->levels[1];
@@ -1169,6 +1426,15 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
                          [&] { sweep(OP_SMOOTH0); }));
         HX_TRY(timed(mx ? "smooth, fp32/bf16" : "smooth, fp64", 0, dinv + vec * 3 + fac + msk,
                      [&] { sweep(OP_SMOOTH); }));
+        if (nb > 1 && mx) {
+            char nm[40];
+            std::snprintf(nm, sizeof nm, "smooth fp32/bf16, %d load cases", nb);
+            HX_TRY(timed(nm, 0, dinv + vec * 3 * nb + fac + msk, [&] {
+                elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, OP_SMOOTH, l0.fac32, l0.mask, (const float*)l0.x,
+                                         (const float*)l0.b, (const uint16_t*)l0.dinv, (float)omega, (float)l0.shift,
+                                         (float*)l0.xt, nullptr, nb);
+            }));
+        }
         HX_TRY(timed(mx ? "residual, fp32" : "residual, fp64", 0, vec * 3 + fac + msk, [&] { sweep(OP_RESID); }));
         const double cvec = 3.0 * es * l1.g.nn;
         HX_TRY(timed(mx ? "restrict, fp32" : "restrict, fp64", 0, vec + cvec + l1.g.nn, [&] {

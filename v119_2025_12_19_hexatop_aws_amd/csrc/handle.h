@@ -10,6 +10,7 @@ namespace hexa {
 
 constexpr int RING = 4;       // pinned state slots for the look-ahead poll
 constexpr int LOOKAHEAD = 2;  // iterations queued beyond the last one inspected
+constexpr int BPART_STRIDE = 4 * RED_BLOCKS;   // partial sums per load case of a batch
 
 inline double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -60,6 +61,14 @@ struct hexa_handle {
     int64_t gj_n = -1;
     const double* gj_A = nullptr;
     hipGraphExec_t filt_exec = nullptr;   // one Helmholtz-filter PCG iteration
+    // one iteration of the batched load-case PCG, for the configuration in
+    // bpcg_key (nb right-hand sides, shift, hierarchy generation)
+    hipGraphExec_t bpcg_exec = nullptr;
+    struct {
+        int nb = -1;
+        double s = 0.0;
+        int64_t gen = -1;
+    } bpcg_key;
     int filt_every = 0;
     bool filt_fused = false;
 
@@ -82,7 +91,9 @@ struct hexa_handle {
     double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_z = nullptr;
     double *d_minv = nullptr, *d_best = nullptr, *d_b = nullptr, *d_diag = nullptr, *d_guess = nullptr;
     double* d_partials = nullptr;
-    double* d_dotpart = nullptr;   // per-block (p, Ap) partials of the fused operator
+    double* d_dotpart = nullptr;   // (p, Ap) partials of the fused operator (per load case)
+    double* d_bpart = nullptr;     // batched PCG: BPART_STRIDE partials per load case
+    double* d_bscr = nullptr;      // batched PCG: RED_BLOCKS alpha range sums per load case
     double* d_scal = nullptr;
     hexa::PcgState* d_state = nullptr;
     hexa::PcgState* h_state = nullptr;   // pinned ring

@@ -75,6 +75,10 @@ inline bool fallback_kernels_forced() {
     return e && e[0] == '1';
 }
 
+// Entries between the right-hand sides of a batch of vectors of n entries
+// (rounded up so that every right-hand side starts 512-byte aligned).
+inline int64_t batch_stride(int64_t n) { return (n + 63) & ~(int64_t)63; }
+
 // Grid description of one level.
 struct Grid3 {
     int nx, ny, nz;     // elements
@@ -144,10 +148,13 @@ void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactC
 // Returns the number of (u, y) partial sums written to dot_part (0 when
 // dot_part is null or the kernel cannot provide them); their values, and so
 // their fixed-order sum, do not depend on the kernel's z-chunk length.
+// nb > 1: nb right-hand sides in one launch, vectors batch_stride(3 nn)
+// apart and partial sums elastic_rw_partials(g) apart; fac, mask and dinv
+// are shared.
 template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
-                      double* dot_part = nullptr);
+                      double* dot_part = nullptr, int nb = 1);
 // Whether the row-wave kernel takes grid g (its buffer descriptors address one
 // node plane of elem_bytes-sized values with 31-bit byte counts); otherwise
 // launch_elastic_rw falls back to the tile kernel, which has no OP_SMOOTH0.
@@ -174,12 +181,14 @@ void launch_coarsen_mask(hipStream_t s, const Grid3& fine, const Grid3& coarse, 
                          uint8_t* cmask);
 
 // Multigrid transfers (k_mg.hip)
+// (nb > 1: a batch of right-hand sides, vectors batch_stride(3 nn) apart on
+// each level)
 template <typename T>
 void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
-                     const uint8_t* cmask, T* rc);
+                     const uint8_t* cmask, T* rc, int nb = 1);
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
-                        const uint8_t* fmask, T* xf);
+                        const uint8_t* fmask, T* xf, int nb = 1);
 // active[e] = (float)rho[e] > thr; RED_BLOCKS per-block counts in partials
 void launch_active_from_rho(hipStream_t s, int64_t ne, const double* rho, float thr, uint8_t* active,
                             double* partials);
@@ -189,7 +198,7 @@ void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, con
                             const uint8_t* mask, double shift, double* A);
 void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf);
 template <typename T>
-void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y);
+void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y, int nb = 1);
 
 // Vectors and reductions (k_vector.hip)
 template <typename T>
@@ -197,7 +206,7 @@ void launch_fill(hipStream_t s, int64_t n, T v, T* x);
 template <typename A, typename B>
 void launch_convert(hipStream_t s, int64_t n, const A* x, B* y);
 template <typename T, typename DT>
-void launch_scale_dinv(hipStream_t s, int64_t n, const T* b, const DT* dinv, T omega, T* x);
+void launch_scale_dinv(hipStream_t s, int64_t n, const T* b, const DT* dinv, T omega, T* x, int nb = 1);
 template <typename DT>
 void launch_make_dinv(hipStream_t s, int64_t n, const double* diag, const uint8_t* mask, double shift,
                       int clamp_mode, DT* dinv);
@@ -265,6 +274,9 @@ void launch_upd_finish(hipStream_t s, int64_t n, const uint8_t* prot, const floa
                        float* dens);
 
 // PCG (k_pcg.hip).  All take the device state and do nothing once it halted.
+// Those with an nb argument run nb right-hand sides in one launch: states
+// st[0..nb), vectors vs entries apart, partial sums ps apart (per right-hand
+// side); nb = 1 is the single solve.
 void launch_pcg_init_state(hipStream_t s, PcgState* st, const double* partials, double tol);
 void launch_pcg_state_for_timing(hipStream_t s, PcgState* st);
 void launch_pcg_rz(hipStream_t s, int64_t n, const double* r, const double* z, double* partials);
@@ -273,24 +285,25 @@ void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, con
                     double* partials);
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check);
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32 = nullptr);
+                       const double* Ap, const double* minv, double* partials, float* r32 = nullptr, int nb = 1,
+                       int64_t vs = 0, int ps = 0);
 // The residual test runs when k % check_every == 0 (check_every > 0), or
 // k == 1 with check_first; k is the state's iteration count.
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int check_every, int check_first,
-                      int cpu_semantics, int jacobi);
+                      int cpu_semantics, int jacobi, int nb = 1, int ps = 0);
 void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
-                          const double* Ap, double* partials);
-void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials);
+                          const double* Ap, double* partials, int nb = 1, int64_t vs = 0, int ps = 0);
+void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials, int nb = 1, int ps = 0);
 void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
-                         const double* r, const double* minv);
+                         const double* r, const double* minv, int nb = 1, int64_t vs = 0);
 // alpha from the n (p, Ap) partials of the operator kernel, summed in a fixed
 // order (through RED_BLOCKS range sums in scratch when n > RED_BLOCKS).
 void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, int breakdown_check,
-                        double* scratch);
+                        double* scratch, int nb = 1, int64_t pstride = 0);
 void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
-                              const double* Ap, double* partials);
+                              const double* Ap, double* partials, int nb = 1, int64_t vs = 0, int ps = 0);
 void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z,
-                             double* x = nullptr);
+                             double* x = nullptr, int nb = 1, int64_t vs = 0);
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best);
 
 // Library-internal entry points shared by the C API files and the job server

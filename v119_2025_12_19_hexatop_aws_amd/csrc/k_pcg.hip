@@ -95,8 +95,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_pAp(int64_t n, const PcgState* 
 
 // alpha = rz / (p, Ap); a breakdown |pAp| < 1e-20 stops the attempt
 // (GPUSolver.jl:373-375).
+// Batched launches (gridDim.y right-hand sides): block row blockIdx.y works
+// on state st + blockIdx.y, vectors vs entries apart and partial sums ps
+// apart; one right-hand side (gridDim.y = 1) uses offsets of 0.
 __global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const double* partials,
-                                                       int breakdown_check) {
+                                                       int breakdown_check, int ps) {
+    st += blockIdx.y;
+    partials += (int64_t)blockIdx.y * ps;
     // the previous iteration's deferred x update (if any) has been applied
     if (threadIdx.x == 0) st->x_done = st->x_iter;
     if (st->halted) return;
@@ -117,7 +122,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha(PcgState* st, const doubl
 // First level of a long fixed-order sum: block b sums the b-th of RED_BLOCKS
 // equal contiguous ranges of the n partials.
 __global__ __launch_bounds__(RED_THREADS) void k_sum_ranges(const PcgState* st, const double* __restrict__ partials,
-                                                            int64_t n, double* __restrict__ out) {
+                                                            int64_t n, double* __restrict__ out, int64_t pin,
+                                                            int pout) {
+    st += blockIdx.y;
+    partials += blockIdx.y * pin;
+    out += (int64_t)blockIdx.y * pout;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const int64_t len = (n + RED_BLOCKS - 1) / RED_BLOCKS;
@@ -130,7 +139,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_sum_ranges(const PcgState* st, 
 
 // alpha from n partials of (p, Ap) written by the operator kernel.
 __global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const double* partials, int n,
-                                                         int breakdown_check) {
+                                                         int breakdown_check, int64_t pin) {
+    st += blockIdx.y;
+    partials += blockIdx.y * pin;
     if (threadIdx.x == 0) st->x_done = st->x_iter;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
@@ -158,7 +169,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st,
                                                         const double* __restrict__ Ap,
                                                         const double* __restrict__ minv,
                                                         double* __restrict__ partials,
-                                                        float* __restrict__ r32) {
+                                                        float* __restrict__ r32, int64_t vs, int ps) {
+    {
+        const int rb = blockIdx.y;
+        st += rb;
+        partials += (int64_t)rb * ps;
+        const int64_t o = rb * vs;
+        if (x) x += o;
+        p += o;
+        r += o;
+        Ap += o;
+        if (r32) r32 += o;
+    }
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double a = st->alpha;
@@ -212,7 +234,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st,
 // (CPUSolver.jl:233-239): rel < tol every iteration, nothing else.
 // For the Jacobi preconditioner beta = (r,z)_new/(r,z) is formed here too.
 __global__ __launch_bounds__(RED_THREADS) void k_check(PcgState* st, const double* partials, int check_every,
-                                                       int check_first, int cpu_semantics, int jacobi) {
+                                                       int check_first, int cpu_semantics, int jacobi, int ps) {
+    st += blockIdx.y;
+    partials += (int64_t)blockIdx.y * ps;
     if (st->halted) return;
     const int k = st->k;
     const bool check = (check_every > 0 && k % check_every == 0) || (check_first && k == 1);
@@ -246,7 +270,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_dots(int64_t n, const PcgS
                                                            const double* __restrict__ r,
                                                            const double* __restrict__ z,
                                                            const double* __restrict__ Ap,
-                                                           double* __restrict__ partials) {
+                                                           double* __restrict__ partials, int64_t vs, int ps) {
+    st += blockIdx.y;
+    partials += (int64_t)blockIdx.y * ps;
+    r += blockIdx.y * vs;
+    z += blockIdx.y * vs;
+    Ap += blockIdx.y * vs;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     double rz = 0.0, zap = 0.0;
@@ -266,7 +295,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_dots(int64_t n, const PcgS
 // Flexible (Polak-Ribiere) beta for a preconditioner that is not exactly a
 // fixed SPD matrix (the mixed-precision V-cycle):
 // beta = (z_new, r_new - r_old)/(z_old, r_old) = -alpha (z_new, Ap)/(z_old, r_old).
-__global__ __launch_bounds__(RED_THREADS) void k_flex_beta(PcgState* st, const double* partials) {
+__global__ __launch_bounds__(RED_THREADS) void k_flex_beta(PcgState* st, const double* partials, int ps) {
+    st += blockIdx.y;
+    partials += (int64_t)blockIdx.y * ps;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     const double rz = final_sum(partials, 0, lds);
@@ -284,7 +315,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_dots_z32(int64_t n, const 
                                                                const double* __restrict__ r,
                                                                const float* __restrict__ z,
                                                                const double* __restrict__ Ap,
-                                                               double* __restrict__ partials) {
+                                                               double* __restrict__ partials, int64_t vs, int ps) {
+    st += blockIdx.y;
+    partials += (int64_t)blockIdx.y * ps;
+    r += blockIdx.y * vs;
+    z += blockIdx.y * vs;
+    Ap += blockIdx.y * vs;
     if (st->halted) return;
     __shared__ double lds[RED_THREADS / 64];
     double rz = 0.0, zap = 0.0;
@@ -307,7 +343,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_flex_dots_z32(int64_t n, const 
 // after a halt neither advance k nor repeat it: the alpha kernel records it
 // as done).
 __global__ void k_p_update_z32(int64_t n, const PcgState* st, double* __restrict__ p, const float* __restrict__ z,
-                               double* __restrict__ x) {
+                               double* __restrict__ x, int64_t vs) {
+    st += blockIdx.y;
+    p += blockIdx.y * vs;
+    z += blockIdx.y * vs;
+    if (x) x += blockIdx.y * vs;
     const bool upx = x && st->x_iter == st->k && st->x_done != st->k;
     const bool upp = !st->halted;
     if (!upx && !upp) return;
@@ -338,7 +378,11 @@ __global__ void k_p_update_z32(int64_t n, const PcgState* st, double* __restrict
 
 // p = z + beta p; with z == nullptr, z = minv * r (Jacobi, not stored).
 __global__ void k_p_update(int64_t n, const PcgState* st, double* __restrict__ p, const double* __restrict__ z,
-                           const double* __restrict__ r, const double* __restrict__ minv) {
+                           const double* __restrict__ r, const double* __restrict__ minv, int64_t vs) {
+    st += blockIdx.y;
+    p += blockIdx.y * vs;
+    if (z) z += blockIdx.y * vs;
+    if (r) r += blockIdx.y * vs;
     if (st->halted) return;
     const double b = st->beta;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -383,42 +427,44 @@ void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, con
     k_pAp<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, p, Ap, partials);
 }
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check) {
-    k_alpha<<<1, RED_THREADS, 0, s>>>(st, partials, breakdown_check);
+    k_alpha<<<1, RED_THREADS, 0, s>>>(st, partials, breakdown_check, 0);
 }
 void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, int breakdown_check,
-                        double* scratch) {
+                        double* scratch, int nb, int64_t pstride) {
     if (n <= RED_BLOCKS) {
-        k_alpha_n<<<1, RED_THREADS, 0, s>>>(st, partials, (int)n, breakdown_check);
+        k_alpha_n<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, partials, (int)n, breakdown_check, pstride);
     } else {
-        k_sum_ranges<<<RED_BLOCKS, RED_THREADS, 0, s>>>(st, partials, n, scratch);
-        k_alpha<<<1, RED_THREADS, 0, s>>>(st, scratch, breakdown_check);
+        k_sum_ranges<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(st, partials, n, scratch, pstride, RED_BLOCKS);
+        k_alpha<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, scratch, breakdown_check, RED_BLOCKS);
     }
 }
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32) {
-    k_update<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32);
+                       const double* Ap, const double* minv, double* partials, float* r32, int nb, int64_t vs,
+                       int ps) {
+    k_update<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32, vs, ps);
 }
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int check_every, int check_first,
-                      int cpu_semantics, int jacobi) {
-    k_check<<<1, RED_THREADS, 0, s>>>(st, partials, check_every, check_first, cpu_semantics, jacobi);
+                      int cpu_semantics, int jacobi, int nb, int ps) {
+    k_check<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, partials, check_every, check_first, cpu_semantics, jacobi, ps);
 }
 void launch_pcg_flex_dots(hipStream_t s, int64_t n, const PcgState* st, const double* r, const double* z,
-                          const double* Ap, double* partials) {
-    k_flex_dots<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials);
+                          const double* Ap, double* partials, int nb, int64_t vs, int ps) {
+    k_flex_dots<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials, vs, ps);
 }
-void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials) {
-    k_flex_beta<<<1, RED_THREADS, 0, s>>>(st, partials);
+void launch_pcg_flex_beta(hipStream_t s, PcgState* st, const double* partials, int nb, int ps) {
+    k_flex_beta<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, partials, ps);
 }
 void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p, const double* z,
-                         const double* r, const double* minv) {
-    k_p_update<<<grid_for(n), 256, 0, s>>>(n, st, p, z, r, minv);
+                         const double* r, const double* minv, int nb, int64_t vs) {
+    k_p_update<<<dim3(grid_for(n), nb), 256, 0, s>>>(n, st, p, z, r, minv, vs);
 }
 void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
-                              const double* Ap, double* partials) {
-    k_flex_dots_z32<<<RED_BLOCKS, RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials);
+                              const double* Ap, double* partials, int nb, int64_t vs, int ps) {
+    k_flex_dots_z32<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(n, st, r, z, Ap, partials, vs, ps);
 }
-void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z, double* x) {
-    k_p_update_z32<<<grid_for((n + 1) / 2), 256, 0, s>>>(n, st, p, z, x);
+void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z, double* x,
+                             int nb, int64_t vs) {
+    k_p_update_z32<<<dim3(grid_for((n + 1) / 2), nb), 256, 0, s>>>(n, st, p, z, x, vs);
 }
 void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const double* x, double* best) {
     k_best_copy<<<grid_for(n), 256, 0, s>>>(n, st, x, best);

@@ -56,7 +56,9 @@ __device__ __forceinline__ float dinv_f<uint16_t>(uint16_t v) { return __uint_as
 
 template <typename T, typename DT>
 __global__ void k_scale_dinv(int64_t n, const T* __restrict__ b, const DT* __restrict__ dinv, T omega,
-                             T* __restrict__ x) {
+                             T* __restrict__ x, int64_t vs) {
+    b += blockIdx.y * vs;   // right-hand side blockIdx.y of a batch (dinv shared)
+    x += blockIdx.y * vs;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         if constexpr (sizeof(DT) == sizeof(double)) x[i] = omega * (T)dinv[i] * b[i];
@@ -66,7 +68,9 @@ __global__ void k_scale_dinv(int64_t n, const T* __restrict__ b, const DT* __res
 template <>
 __global__ void k_scale_dinv<double, double>(int64_t n, const double* __restrict__ b,
                                              const double* __restrict__ dinv, double omega,
-                                             double* __restrict__ x) {
+                                             double* __restrict__ x, int64_t vs) {
+    b += blockIdx.y * vs;
+    x += blockIdx.y * vs;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         x[i] = omega * dinv[i] * b[i];
@@ -171,8 +175,8 @@ void launch_convert(hipStream_t s, int64_t n, const A* x, B* y) {
     k_convert<A, B><<<grid_for(n), 256, 0, s>>>(n, x, y);
 }
 template <typename T, typename DT>
-void launch_scale_dinv(hipStream_t s, int64_t n, const T* b, const DT* dinv, T omega, T* x) {
-    k_scale_dinv<T, DT><<<grid_for(n), 256, 0, s>>>(n, b, dinv, omega, x);
+void launch_scale_dinv(hipStream_t s, int64_t n, const T* b, const DT* dinv, T omega, T* x, int nb) {
+    k_scale_dinv<T, DT><<<dim3(grid_for(n), nb), 256, 0, s>>>(n, b, dinv, omega, x, batch_stride(n));
 }
 template <typename DT>
 void launch_make_dinv(hipStream_t s, int64_t n, const double* diag, const uint8_t* mask, double shift,
@@ -201,11 +205,11 @@ template void launch_fill<uint8_t>(hipStream_t, int64_t, uint8_t, uint8_t*);
 template void launch_convert<double, float>(hipStream_t, int64_t, const double*, float*);
 template void launch_convert<float, double>(hipStream_t, int64_t, const float*, double*);
 template void launch_scale_dinv<double, double>(hipStream_t, int64_t, const double*, const double*,
-                                                double, double*);
+                                                double, double*, int);
 template void launch_scale_dinv<float, float>(hipStream_t, int64_t, const float*, const float*, float,
-                                              float*);
+                                              float*, int);
 template void launch_scale_dinv<float, uint16_t>(hipStream_t, int64_t, const float*, const uint16_t*,
-                                                 float, float*);
+                                                 float, float*, int);
 template void launch_make_dinv<double>(hipStream_t, int64_t, const double*, const uint8_t*, double, int,
                                        double*);
 template void launch_make_dinv<float>(hipStream_t, int64_t, const double*, const uint8_t*, double, int,

@@ -34,6 +34,7 @@ PRECOND_JACOBI, PRECOND_MG, PRECOND_REF_MG = 0, 1, 2
 MATVEC_AUTO, MATVEC_DENSE, MATVEC_FACTORED = 0, 1, 2
 SEMANTICS_GPU, SEMANTICS_CPU = 0, 1
 DIAG_EXACT, DIAG_X_ENTRY = 0, 1
+RHS_AUTO, RHS_SEQUENTIAL = 0, 1
 METHOD_NAMES = {0: "Jacobi_GPU", 1: "GMG_GPU", 2: "Failed_Corrupt_Input", 3: "Jacobi_CPU", 4: "None"}
 
 
@@ -60,7 +61,8 @@ class SolveParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("shift_factor", C.c_double),
                 ("max_shift_attempts", C.c_int32), ("shift_multiplier", C.c_double),
                 ("stagnation_tol", C.c_double), ("precond", C.c_int32), ("semantics", C.c_int32),
-                ("warm_start", C.c_int32), ("check_every", C.c_int32), ("reserved", C.c_int64 * 4)]
+                ("warm_start", C.c_int32), ("check_every", C.c_int32), ("rhs_mode", C.c_int32),
+                ("pad", C.c_int32), ("reserved", C.c_int64 * 3)]
 
 
 class SolveReport(C.Structure):
