@@ -141,6 +141,51 @@ def test_sensitivity_and_oc_update_match_oracle():
     assert abs(w @ xg - 0.3 * p.n_elem) < 1e-8 * p.n_elem
 
 
+@pytest.mark.parametrize("kernel", [hexa.MATVEC_FACTORED, hexa.MATVEC_DENSE])
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 1.0, 1.5), (2.0, 0.7, 1.0)])
+def test_sensitivity_factored_and_dense_element_forms(h, kernel):
+    """The element energy u_e' K0 u_e from the factored (modal) form of the
+    brick stiffness and from the dense 24x24 matrix, on non-cubic elements.
+
+    The dense sum carries rounding of order eps |K0| |u_e|^2, so a large
+    rigid translation of an element (|u| near 280 at h = (2, 0.7, 1)) costs
+    it digits; the modal form drops the translation mode exactly.  Both are
+    checked against the energy of u_e minus its corner mean (the same energy,
+    since K0 annihilates translations) summed in extended precision, and the
+    dense kernel also against the oracle's dense sum (the two round the same
+    terms in different orders: about 3e-12 of max |dc| each at h = (2, 0.7, 1),
+    where the modal form is within 4e-14)."""
+    say("sensitivity forms", h, kernel)
+    p = P.cantilever(13, 7, 9, seed=23, lo=0.1)
+    Ke = oracle.ke_hex8(*h, nu=0.3)
+    with hexa.Solver(*p.grid, h, E0=1.0, Emin=1e-9, penal=3.0, matvec_kernel=kernel) as s:
+        s.set_problem(p.fixed, p.f[0])
+        s.set_density(p.rho)
+        u, reps, st = s.solve(tol=1e-8, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU)
+        assert st == hexa.OK
+        c, dc = s.sensitivity()
+    co, dco = oracle.simp_sensitivity(p.grid, Ke, p.rho, 1.0, 1e-9, 3.0, u[0])
+    nx, ny, nz = p.grid
+    e = np.arange(p.n_elem)
+    i, j, k = e % nx, (e // nx) % ny, e // (nx * ny)
+    cx, cy, cz = [0, 1, 1, 0, 0, 1, 1, 0], [0, 0, 1, 1, 0, 0, 1, 1], [0, 0, 0, 0, 1, 1, 1, 1]
+    nodes = np.stack([(i + cx[a]) + (nx + 1) * ((j + cy[a]) + (ny + 1) * (k + cz[a])) for a in range(8)], 1)
+    ue = u[0].reshape(-1, 3)[nodes]
+    ue = (ue - ue.mean(axis=1, keepdims=True)).reshape(-1, 24).astype(np.longdouble)
+    ce = np.einsum("ei,ij,ej->e", ue, Ke.astype(np.longdouble), ue)
+    rl = p.rho.astype(np.longdouble)
+    dref = (-3.0 * rl ** 2 * (1.0 - 1e-9) * ce).astype(np.float64)
+    cref = float(np.sum((1e-9 + rl ** 3 * (1.0 - 1e-9)) * ce))
+    err = np.abs(dc - dref).max() / np.abs(dref).max()
+    print("max |dc - dref| / max |dref| =", err, " oracle:", np.abs(dco - dref).max() / np.abs(dref).max(),
+          " |c - cref| / cref =", abs(c - cref) / cref, " oracle:", abs(co - cref) / cref)
+    tol = 1e-12 if kernel == hexa.MATVEC_FACTORED else 2e-11
+    assert err < tol and abs(c - cref) < tol * cref
+    assert abs(c - co) < 2e-11 * co
+    if kernel == hexa.MATVEC_DENSE:
+        assert np.abs(dc - dco).max() < 1e-11 * np.abs(dco).max()
+
+
 def oracle_oc_loop(p, iters, rmin=1.5, volfrac=0.3, move=0.2, tol=1e-10):
     Ke = oracle.ke_hex8(1, 1, 1, 0.3)
     x = p.rho.copy()

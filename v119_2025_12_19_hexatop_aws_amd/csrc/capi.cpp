@@ -1487,8 +1487,12 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
                       h->d_partials, h->d_scal + 8);
         }));
         HX_TRY(timed("SIMP sensitivity", 0, 24.0 * nn + 16.0 * ne, [&] {
-            launch_simp_sens(st, g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, e[1],
-                             h->d_partials);
+            if (h->factored)
+                launch_simp_sens_fact(st, g, h->coef64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, e[1],
+                                      h->d_partials);
+            else
+                launch_simp_sens(st, g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, e[1],
+                                 h->d_partials);
         }));
     }
     HX_CHECK(hipStreamSynchronize(st));

@@ -341,8 +341,12 @@ int32_t hexa_sensitivity(hexa_handle* h, double* dc, double* compliance) {
     if (!h->have_solution || !h->have_density) { set_error("solve first"); return HEXA_E_INVALID; }
     HX_TRY(ensure_elem(h));
     hipStream_t st = h->stream;
-    launch_simp_sens(st, h->g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, h->d_e2,
-                     h->d_partials);
+    if (h->factored)
+        launch_simp_sens_fact(st, h->g, h->coef64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, h->d_e2,
+                              h->d_partials);
+    else
+        launch_simp_sens(st, h->g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, h->d_e2,
+                         h->d_partials);
     launch_final_sum(st, 1, h->d_partials, h->d_scal);
     HX_TRY(check_launch());
     double c = 0.0;
@@ -417,8 +421,12 @@ int32_t oc_step_core(hexa_handle* h, const hexa_oc_params* p, hexa_oc_report* re
         return ss;
     }
     // dc~ and compliance, dc = F^T dc~ = F dc~, dv = F^T 1 = w
-    launch_simp_sens(st, h->g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, h->d_e2,
-                     h->d_partials);
+    if (h->factored)
+        launch_simp_sens_fact(st, h->g, h->coef64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, h->d_e2,
+                              h->d_partials);
+    else
+        launch_simp_sens(st, h->g, h->d_Ke64, h->d_rho, h->opt.E0, h->opt.Emin, h->opt.penal, h->d_x, h->d_e2,
+                         h->d_partials);
     launch_final_sum(st, 1, h->d_partials, h->d_scal + 4);
     HX_TRY(check_launch());
     HX_TRY(filter_dev(h, h->d_e2, h->d_e3, p->rmin, p->filter_tol, p->filter_max_iter, 1, &fit, &frel, true));

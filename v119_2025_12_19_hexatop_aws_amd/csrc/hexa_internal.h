@@ -244,6 +244,9 @@ void launch_stress(hipStream_t s, const Grid3& g, double dx, double dy, double d
                    float* dmin, float* voigt);
 void launch_simp_sens(hipStream_t s, const Grid3& g, const double* Ke, const double* rho, double E0, double Emin,
                       double penal, const double* U, double* dc, double* partials);
+// The same with the exact brick stiffness in factored form (coef at E = 1).
+void launch_simp_sens_fact(hipStream_t s, const Grid3& g, const FactCoef<double>& coef, const double* rho, double E0,
+                           double Emin, double penal, const double* U, double* dc, double* partials);
 int oc_bracket_bytes();
 void launch_oc(hipStream_t s, int64_t n, const double* dc, const double* dv, const double* w, double volfrac,
                double move, double tol, int max_passes, double* x, double* q, double* lo, double* hi, void* bracket,

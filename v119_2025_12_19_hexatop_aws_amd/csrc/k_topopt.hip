@@ -160,6 +160,71 @@ __global__ __launch_bounds__(RED_THREADS) void k_simp_sens(Grid3 g, const double
     if (threadIdx.x == 0) partials[blockIdx.x] = csum;
 }
 
+// The same for the exact brick stiffness in its factored form (the row-wave
+// operator's): u_e' K0 u_e = v' M v with v the per-axis sum/difference modes
+// of the element's corner values and M the modal diagonal plus the
+// cross-component couplings, about 100 flops per element instead of 1152.
+__global__ __launch_bounds__(RED_THREADS) void k_simp_sens_fact(Grid3 g, FactCoef<double> coef,
+                                                                const double* __restrict__ rho, double E0, double Emin,
+                                                                double penal, const double* __restrict__ U,
+                                                                double* __restrict__ dc, double* __restrict__ partials) {
+    __shared__ double lds[RED_THREADS / 64];
+    double csum = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x; e < g.ne; e += (int64_t)RED_BLOCKS * RED_THREADS) {
+        double ue[24];
+        elem_disp(U, g, e, ue);
+        // modes m = mx + 2 my + 4 mz (bit set = difference along that axis);
+        // corners n1..n8 = (0,0,0) (1,0,0) (1,1,0) (0,1,0), then z = 1
+        double v[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double P[2][4];
+#pragma unroll
+            for (int z = 0; z < 2; ++z) {
+                const double a = ue[3 * (4 * z + 0) + c], bx = ue[3 * (4 * z + 1) + c];
+                const double d = ue[3 * (4 * z + 2) + c], cy = ue[3 * (4 * z + 3) + c];
+                const double s0 = a + bx, d0 = bx - a, s1 = cy + d, d1 = d - cy;
+                P[z][0] = s0 + s1;
+                P[z][1] = d0 + d1;
+                P[z][2] = s1 - s0;
+                P[z][3] = d1 - d0;
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                v[c][m] = P[0][m] + P[1][m];
+                v[c][m + 4] = P[1][m] - P[0][m];
+            }
+        }
+        double ce = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double yv[8];
+#pragma unroll
+            for (int m = 1; m < 8; ++m) yv[m] = coef.D[c][m] * v[c][m];
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) {
+                if (bb == c) continue;
+                const int d = 3 - c - bb;
+#pragma unroll
+                for (int qq = 0; qq < 2; ++qq) {
+                    const int mq = qq << d;
+                    yv[(1 << c) | mq] += coef.cl[d][qq] * v[bb][(1 << bb) | mq];
+                    yv[(1 << bb) | mq] += coef.cm[d][qq] * v[bb][(1 << c) | mq];
+                }
+            }
+#pragma unroll
+            for (int m = 1; m < 8; ++m) ce += v[c][m] * yv[m];
+        }
+        const double rp = rho[e];
+        const double pw = penal == 3.0 ? rp * rp * rp : pow(rp, penal);
+        const double pm = penal == 3.0 ? rp * rp : pow(rp, penal - 1.0);
+        csum += (Emin + pw * (E0 - Emin)) * ce;
+        dc[e] = -penal * pm * (E0 - Emin) * ce;
+    }
+    csum = block_sum(csum, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = csum;
+}
+
 // ---------------------------------------------------------------- OC
 // Multisection on the Lagrange multiplier: each pass evaluates the filtered
 // volume at OC_NL points inside the bracket and keeps the sub-interval where
@@ -521,6 +586,10 @@ void launch_stress(hipStream_t s, const Grid3& g, double dx, double dy, double d
 void launch_simp_sens(hipStream_t s, const Grid3& g, const double* Ke, const double* rho, double E0, double Emin,
                       double penal, const double* U, double* dc, double* partials) {
     k_simp_sens<<<RED_BLOCKS, RED_THREADS, 0, s>>>(g, Ke, rho, E0, Emin, penal, U, dc, partials);
+}
+void launch_simp_sens_fact(hipStream_t s, const Grid3& g, const FactCoef<double>& coef, const double* rho, double E0,
+                           double Emin, double penal, const double* U, double* dc, double* partials) {
+    k_simp_sens_fact<<<RED_BLOCKS, RED_THREADS, 0, s>>>(g, coef, rho, E0, Emin, penal, U, dc, partials);
 }
 int oc_bracket_bytes() { return (int)sizeof(OcBracket); }
 void launch_oc(hipStream_t s, int64_t n, const double* dc, const double* dv, const double* w, double volfrac,
