@@ -224,8 +224,8 @@ int32_t mg_build(hexa_handle* h) {
         return HEXA_E_INVALID;
     }
     HX_TRY(dalloc(h, &h->d_Ainv, (size_t)(h->n_coarse * h->n_coarse)));
-    HX_TRY(dalloc(h, &h->d_gj_row, (size_t)h->n_coarse));
-    HX_TRY(dalloc(h, &h->d_gj_col, (size_t)h->n_coarse));
+    HX_TRY(dalloc(h, &h->d_gj_row, (size_t)gj_row_doubles(h->n_coarse)));
+    HX_TRY(dalloc(h, &h->d_gj_col, (size_t)gj_col_doubles(h->n_coarse)));
     return HEXA_OK;
 }
 
@@ -249,8 +249,8 @@ int32_t mg_set_shift(hexa_handle* h, double s) {
     Level& c = h->levels[L - 1];
     launch_fill<double>(h->stream, h->n_coarse * h->n_coarse, 0.0, h->d_Ainv);
     launch_coarse_assemble(h->stream, c.g, c.Ke64, c.fac64, c.mask, c.shift, h->d_Ainv);
-    // the sweep's 2 n launches replay from a graph captured once per handle
-    // and coarse size (launching them one by one costs more than running them)
+    // the inversion's launches replay from a graph captured once per handle
+    // and coarse size
     const char* ge = getenv("HEXA_GRAPHS");
     if (ge && ge[0] == '0') {
         launch_gauss_jordan(h->stream, h->n_coarse, h->d_Ainv, h->d_gj_row, h->d_gj_col);

@@ -196,6 +196,11 @@ void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, c
                         const double* kd8, double* dcinv);
 void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,
                             const uint8_t* mask, double shift, double* A);
+// In-place inverse of the SPD n x n matrix A; rowbuf holds gj_row_doubles(n)
+// and colbuf gj_col_doubles(n) values.
+constexpr int GJ_PANEL = 32;
+inline int64_t gj_row_doubles(int64_t n) { return GJ_PANEL * n + GJ_PANEL * GJ_PANEL; }
+inline int64_t gj_col_doubles(int64_t n) { return GJ_PANEL * n; }
 void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf);
 template <typename T>
 void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y, int nb = 1);

@@ -77,8 +77,7 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
     // k+2, in flight while plane k is computed.  Four steps rotate the four
     // buffers back to their roles, so no register is copied across the loop
     // (a copy of an in-flight load would make the loop wait for it).  Planes
-    // past the chunk are loaded and computed but not stored: their stores
-    // go through a descriptor of zero size.
+    // outside the box read 0 (prefetches past the last plane).
     auto step = [&](const double (&a)[9], const double (&b)[9], const double (&c)[9], double (&d)[9], int k) {
         load(k + 2, d);
         const double* w[3] = {a, b, c};
@@ -112,10 +111,15 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
     load(k0 - 1, wa);
     load(k0, wb);
     load(k0 + 1, wc);
+    // (a chunk shorter than the four-step trip stops after its last plane:
+    // small grids run chunks of one or two planes to fill the GPU)
     for (int k = k0; k < k1; k += 4) {
         step(wa, wb, wc, wd, k);
+        if (k + 1 >= k1) break;
         step(wb, wc, wd, wa, k + 1);
+        if (k + 2 >= k1) break;
         step(wc, wd, wa, wb, k + 2);
+        if (k + 3 >= k1) break;
         step(wd, wa, wb, wc, k + 3);
     }
     if (PAP) {

@@ -262,30 +262,93 @@ __global__ void k_coarse_assemble(Grid3 g, const double* __restrict__ Ke, const 
     }
 }
 
-// In-place Gauss-Jordan inversion of an SPD matrix, one pivot per pair of
-// launches (no pivoting needed for SPD): save pivot row/column, then update.
-__global__ void k_gj_save(int64_t n, int64_t k, const double* __restrict__ A, double* __restrict__ rowb,
-                          double* __restrict__ colb) {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        rowb[t] = A[k * n + t];
-        colb[t] = A[t * n + k];
+// In-place blocked Gauss-Jordan inversion of an SPD matrix (no pivoting is
+// needed: every Schur complement of an SPD matrix is SPD), GJ_PANEL pivots
+// per pair of launches.  With the pivot block p (rows and columns
+// p0 .. p0+nb-1) and P = inv(A_pp):
+//   A_ij <- A_ij - A_ip P A_pj   (i, j outside p)
+//   A_pj <- P A_pj,  A_ip <- -A_ip P,  A_pp <- P.
+// The panel kernel (one workgroup) inverts A_pp by scalar Gauss-Jordan in
+// LDS and writes P, R = P A_p. (rows, GJ_PANEL x n) and a copy C of the
+// column panel A_.p (n x GJ_PANEL); the update kernel then rewrites every
+// entry from C, R and P in 32 x 32 tiles, each entry's sum in a fixed order.
+// A 4131-DOF coarse grid takes 2 x 130 launches instead of 2 x 4131.
+constexpr int GJ_NB = GJ_PANEL;
+
+__global__ __launch_bounds__(GJ_NB* GJ_NB) void k_bgj_panel(int64_t n, int64_t p0, int nb,
+                                                          const double* __restrict__ A, double* __restrict__ R,
+                                                          double* __restrict__ C, double* __restrict__ P) {
+    __shared__ double S[GJ_NB][GJ_NB + 1];
+    __shared__ double rk[GJ_NB], ck[GJ_NB];
+    const int t = threadIdx.x, r = t / GJ_NB, c = t % GJ_NB;
+    const bool in = r < nb && c < nb;
+    S[r][c] = in ? A[(p0 + r) * n + p0 + c] : 0.0;
+    __syncthreads();
+    for (int k = 0; k < nb; ++k) {
+        if (t < GJ_NB) {
+            rk[t] = S[k][t];
+            ck[t] = S[t][k];
+        }
+        __syncthreads();
+        if (in) {
+            const double piv = 1.0 / rk[k];
+            double v;
+            if (r == k && c == k) v = piv;
+            else if (r == k) v = rk[c] * piv;
+            else if (c == k) v = -ck[r] * piv;
+            else v = S[r][c] - ck[r] * rk[c] * piv;
+            S[r][c] = v;
+        }
+        __syncthreads();
+    }
+    P[t] = in ? S[r][c] : 0.0;
+    for (int64_t e = t; e < n * GJ_NB; e += GJ_NB * GJ_NB) {
+        const int64_t i = e / GJ_NB;
+        const int q = (int)(e - i * GJ_NB);
+        C[e] = q < nb ? A[i * n + p0 + q] : 0.0;
+    }
+    for (int64_t e = t; e < n * GJ_NB; e += GJ_NB * GJ_NB) {
+        const int q = (int)(e / n);
+        const int64_t j = e - q * n;
+        double acc = 0.0;
+        if (q < nb)
+            for (int m = 0; m < nb; ++m) acc = __builtin_fma(S[q][m], A[(p0 + m) * n + j], acc);
+        R[e] = acc;
     }
 }
 
-__global__ void k_gj_update(int64_t n, int64_t k, double* __restrict__ A, const double* __restrict__ rowb,
-                            const double* __restrict__ colb) {
-    const double piv = 1.0 / rowb[k];
-    const int64_t nn = n * n;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = t / n, j = t % n;
+__global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t p0, int nb, double* __restrict__ A,
+                                                    const double* __restrict__ R, const double* __restrict__ C,
+                                                    const double* __restrict__ P) {
+    __shared__ double Cs[GJ_NB][GJ_NB + 1], Rs[GJ_NB][GJ_NB + 1], Ps[GJ_NB][GJ_NB + 1];
+    const int64_t i0 = (int64_t)blockIdx.y * GJ_NB, j0 = (int64_t)blockIdx.x * GJ_NB;
+    for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += 256) {
+        const int a = e / GJ_NB, b = e % GJ_NB;
+        Cs[a][b] = i0 + a < n ? C[(i0 + a) * GJ_NB + b] : 0.0;
+        Rs[a][b] = j0 + b < n ? R[a * n + j0 + b] : 0.0;
+        Ps[a][b] = P[e];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += 256) {
+        const int a = e / GJ_NB, b = e % GJ_NB;
+        const int64_t i = i0 + a, j = j0 + b;
+        if (i >= n || j >= n) continue;
+        const bool ip = i >= p0 && i < p0 + nb, jp = j >= p0 && j < p0 + nb;
         double v;
-        if (i == k && j == k) v = piv;
-        else if (i == k) v = rowb[j] * piv;
-        else if (j == k) v = -colb[i] * piv;
-        else v = A[t] - colb[i] * rowb[j] * piv;
-        A[t] = v;
+        if (ip && jp) {
+            v = Ps[i - p0][j - p0];
+        } else if (ip) {
+            v = Rs[i - p0][b];
+        } else if (jp) {
+            double acc = 0.0;
+            for (int m = 0; m < nb; ++m) acc = __builtin_fma(Cs[a][m], Ps[m][j - p0], acc);
+            v = -acc;
+        } else {
+            double acc = 0.0;
+            for (int m = 0; m < nb; ++m) acc = __builtin_fma(Cs[a][m], Rs[m][b], acc);
+            v = A[i * n + j] - acc;
+        }
+        A[i * n + j] = v;
     }
 }
 
@@ -339,10 +402,12 @@ void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, con
 }
 
 void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf) {
-    const unsigned gs = grid_for(n), gu = grid_for(n * n);
-    for (int64_t k = 0; k < n; ++k) {
-        k_gj_save<<<gs, 256, 0, s>>>(n, k, A, rowbuf, colbuf);
-        k_gj_update<<<gu, 256, 0, s>>>(n, k, A, rowbuf, colbuf);
+    const unsigned tiles = (unsigned)((n + GJ_NB - 1) / GJ_NB);
+    double* P = rowbuf + (int64_t)GJ_NB * n;
+    for (int64_t p0 = 0; p0 < n; p0 += GJ_NB) {
+        const int nb = (int)std::min<int64_t>(GJ_NB, n - p0);
+        k_bgj_panel<<<1, GJ_NB * GJ_NB, 0, s>>>(n, p0, nb, A, rowbuf, colbuf, P);
+        k_bgj_update<<<dim3(tiles, tiles), 256, 0, s>>>(n, p0, nb, A, rowbuf, colbuf, P);
     }
 }
 

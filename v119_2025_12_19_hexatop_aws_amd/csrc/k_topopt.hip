@@ -225,6 +225,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_simp_sens_fact(Grid3 g, FactCoe
     if (threadIdx.x == 0) partials[blockIdx.x] = csum;
 }
 
+// OC_NL block sums at once, each in block_sum's order (wave butterfly, then
+// the waves in order), with one barrier instead of one per sum: on return
+// thread k < OC_NL holds sum k in v[0].
+__device__ __forceinline__ void block_sums(double (&v)[OC_NL], double (&lw)[RED_THREADS / 64][OC_NL]) {
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < OC_NL; ++k) {
+        double t = v[k];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if ((threadIdx.x & 63) == 0) lw[w][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < OC_NL) {
+        double t = 0.0;
+        for (int i = 0; i < RED_THREADS / 64; ++i) t += lw[i][threadIdx.x];
+        v[0] = t;
+    }
+}
+
 // ---------------------------------------------------------------- OC
 // Multisection on the Lagrange multiplier: each pass evaluates the filtered
 // volume at OC_NL points inside the bracket and keeps the sub-interval where
@@ -273,7 +292,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_oc_eval(int64_t n, const OcBrac
                                                          double move, const double* __restrict__ w,
                                                          double* __restrict__ partials) {
     if (br->done) return;
-    __shared__ double lds[RED_THREADS / 64];
+    __shared__ double lw[RED_THREADS / 64][OC_NL];
     const double l1 = br->l1, l2 = br->l2;
     double s[OC_NL];
 #pragma unroll
@@ -288,23 +307,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_oc_eval(int64_t n, const OcBrac
 #pragma unroll
         for (int k = 0; k < OC_NL; ++k) s[k] = fma(we, fmin(fmax(qe * inv[k], le), he), s[k]);
     }
-#pragma unroll
-    for (int k = 0; k < OC_NL; ++k) {
-        const double t = block_sum(s[k], lds);
-        if (threadIdx.x == 0) partials[k * RED_BLOCKS + blockIdx.x] = t;
-    }
+    block_sums(s, lw);
+    if (threadIdx.x < OC_NL) partials[threadIdx.x * RED_BLOCKS + blockIdx.x] = s[0];
 }
 
 __global__ __launch_bounds__(RED_THREADS) void k_oc_pick(OcBracket* br, const double* __restrict__ partials) {
     if (br->done) return;
-    __shared__ double lds[RED_THREADS / 64];
+    __shared__ double lw[RED_THREADS / 64][OC_NL];
     __shared__ double vol[OC_NL];
+    double s[OC_NL];
+#pragma unroll
     for (int k = 0; k < OC_NL; ++k) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS) s += partials[k * RED_BLOCKS + i];
-        s = block_sum(s, lds);
-        if (threadIdx.x == 0) vol[k] = s;
+        s[k] = 0.0;
+        for (int i = threadIdx.x; i < RED_BLOCKS; i += RED_THREADS) s[k] += partials[k * RED_BLOCKS + i];
     }
+    block_sums(s, lw);
+    if (threadIdx.x < OC_NL) vol[threadIdx.x] = s[0];
+    __syncthreads();
     if (threadIdx.x != 0) return;
     const double l1 = br->l1, l2 = br->l2;
     // volume decreases with lambda: new bracket around the first point whose
