@@ -264,20 +264,24 @@ __global__ void k_coarse_assemble(Grid3 g, const double* __restrict__ Ke, const 
 
 // In-place blocked Gauss-Jordan inversion of an SPD matrix (no pivoting is
 // needed: every Schur complement of an SPD matrix is SPD), GJ_PANEL pivots
-// per pair of launches.  With the pivot block p (rows and columns
+// per three launches.  With the pivot block p (rows and columns
 // p0 .. p0+nb-1) and P = inv(A_pp):
 //   A_ij <- A_ij - A_ip P A_pj   (i, j outside p)
 //   A_pj <- P A_pj,  A_ip <- -A_ip P,  A_pp <- P.
-// The panel kernel (one workgroup) inverts A_pp by scalar Gauss-Jordan in
-// LDS and writes P, R = P A_p. (rows, GJ_PANEL x n) and a copy C of the
-// column panel A_.p (n x GJ_PANEL); the update kernel then rewrites every
-// entry from C, R and P in 32 x 32 tiles, each entry's sum in a fixed order.
-// A 4131-DOF coarse grid takes 2 x 130 launches instead of 2 x 4131.
+//  * k_bgj_pivot (one workgroup): P by scalar Gauss-Jordan in LDS;
+//  * k_bgj_panels (one workgroup per 32 rows / columns): R = P A_p. (the
+//    block row, GJ_PANEL x n) and a copy C = A_.p (the block column,
+//    n x GJ_PANEL), both read before the update overwrites them;
+//  * k_bgj_update (one workgroup per 32 x 32 tile): outside the block row
+//    and column, A_ij - (C R)_ij, a rank-32 update on the FP64 matrix cores
+//    (each wave one 16 x 16 quarter, 8 steps of v_mfma_f64_16x16x4); the
+//    block row and column tiles entry by entry.
+// A 4131-DOF coarse grid takes 3 x 130 launches instead of 2 x 4131.
 constexpr int GJ_NB = GJ_PANEL;
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(GJ_NB* GJ_NB) void k_bgj_panel(int64_t n, int64_t p0, int nb,
-                                                          const double* __restrict__ A, double* __restrict__ R,
-                                                          double* __restrict__ C, double* __restrict__ P) {
+__global__ __launch_bounds__(GJ_NB* GJ_NB) void k_bgj_pivot(int64_t n, int64_t p0, int nb,
+                                                          const double* __restrict__ A, double* __restrict__ P) {
     __shared__ double S[GJ_NB][GJ_NB + 1];
     __shared__ double rk[GJ_NB], ck[GJ_NB];
     const int t = threadIdx.x, r = t / GJ_NB, c = t % GJ_NB;
@@ -302,18 +306,27 @@ __global__ __launch_bounds__(GJ_NB* GJ_NB) void k_bgj_panel(int64_t n, int64_t p
         __syncthreads();
     }
     P[t] = in ? S[r][c] : 0.0;
-    for (int64_t e = t; e < n * GJ_NB; e += GJ_NB * GJ_NB) {
-        const int64_t i = e / GJ_NB;
-        const int q = (int)(e - i * GJ_NB);
-        C[e] = q < nb ? A[i * n + p0 + q] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_bgj_panels(int64_t n, int64_t p0, int nb, const double* __restrict__ A,
+                                                    const double* __restrict__ P, double* __restrict__ R,
+                                                    double* __restrict__ C) {
+    __shared__ double Ps[GJ_NB][GJ_NB + 1], As[GJ_NB][GJ_NB + 1];
+    const int64_t t0 = (int64_t)blockIdx.x * GJ_NB;
+    for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += 256) {
+        const int a = e / GJ_NB, b = e % GJ_NB;
+        Ps[a][b] = P[e];
+        As[a][b] = (a < nb && t0 + b < n) ? A[(p0 + a) * n + t0 + b] : 0.0;
+        // block column: rows t0.., columns p0..
+        if (t0 + a < n) C[(t0 + a) * GJ_NB + b] = b < nb ? A[(t0 + a) * n + p0 + b] : 0.0;
     }
-    for (int64_t e = t; e < n * GJ_NB; e += GJ_NB * GJ_NB) {
-        const int q = (int)(e / n);
-        const int64_t j = e - q * n;
+    __syncthreads();
+    for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += 256) {
+        const int q = e / GJ_NB, b = e % GJ_NB;
+        if (t0 + b >= n) continue;
         double acc = 0.0;
-        if (q < nb)
-            for (int m = 0; m < nb; ++m) acc = __builtin_fma(S[q][m], A[(p0 + m) * n + j], acc);
-        R[e] = acc;
+        for (int m = 0; m < nb; ++m) acc = __builtin_fma(Ps[q][m], As[m][b], acc);
+        R[q * n + t0 + b] = acc;
     }
 }
 
@@ -322,13 +335,30 @@ __global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t p0, int n
                                                     const double* __restrict__ P) {
     __shared__ double Cs[GJ_NB][GJ_NB + 1], Rs[GJ_NB][GJ_NB + 1], Ps[GJ_NB][GJ_NB + 1];
     const int64_t i0 = (int64_t)blockIdx.y * GJ_NB, j0 = (int64_t)blockIdx.x * GJ_NB;
+    const bool pivot_tile = i0 == p0 || j0 == p0;   // p0 is a multiple of GJ_NB
     for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += 256) {
         const int a = e / GJ_NB, b = e % GJ_NB;
         Cs[a][b] = i0 + a < n ? C[(i0 + a) * GJ_NB + b] : 0.0;
         Rs[a][b] = j0 + b < n ? R[a * n + j0 + b] : 0.0;
-        Ps[a][b] = P[e];
+        if (pivot_tile) Ps[a][b] = P[e];
     }
     __syncthreads();
+    if (!pivot_tile) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int wr = 16 * (w >> 1), wc = 16 * (w & 1), lj = lane & 15, lk = lane >> 4;
+        gj_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int st = 0; st < GJ_NB / 4; ++st)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[wr + lj][4 * st + lk], Rs[4 * st + lk][wc + lj], acc, 0, 0,
+                                                       0);
+        // accumulator r of lane group lk holds row lk + 4 r, column lj
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t i = i0 + wr + lk + 4 * r, j = j0 + wc + lj;
+            if (i < n && j < n) A[i * n + j] -= acc[r];
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < GJ_NB * GJ_NB; e += 256) {
         const int a = e / GJ_NB, b = e % GJ_NB;
         const int64_t i = i0 + a, j = j0 + b;
@@ -406,7 +436,8 @@ void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, do
     double* P = rowbuf + (int64_t)GJ_NB * n;
     for (int64_t p0 = 0; p0 < n; p0 += GJ_NB) {
         const int nb = (int)std::min<int64_t>(GJ_NB, n - p0);
-        k_bgj_panel<<<1, GJ_NB * GJ_NB, 0, s>>>(n, p0, nb, A, rowbuf, colbuf, P);
+        k_bgj_pivot<<<1, GJ_NB * GJ_NB, 0, s>>>(n, p0, nb, A, P);
+        k_bgj_panels<<<tiles, 256, 0, s>>>(n, p0, nb, A, P, rowbuf, colbuf);
         k_bgj_update<<<dim3(tiles, tiles), 256, 0, s>>>(n, p0, nb, A, rowbuf, colbuf, P);
     }
 }
