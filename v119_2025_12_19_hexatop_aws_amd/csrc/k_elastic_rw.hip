@@ -166,8 +166,15 @@ __device__ __forceinline__ T ld1_dinv(Rsrc r, uint32_t off, const uint16_t*) {
     return (T)__uint_as_float(((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0)) << 16);
 }
 
+// Occupancy: the FP64 kernels hold 232-256 VGPRs, 2 waves per SIMD (one
+// 8-wave block per CU).  The FP32 kernels are held to 128 VGPRs, 4 waves per
+// SIMD, so that two blocks share a CU: the first smoothing sweep needed
+// 134-138 VGPRs and ran one block per CU (0.547 ms at 512x256x256, 0.449
+// held to 128; its BF16-diagonal form spills 12 bytes per lane).  The FP64
+// code is the same as with the default bound.
 template <typename T, typename DT, int MODE>
-__global__ __launch_bounds__(WL* BY) void k_elastic_rw(Grid3 g, FactCoef<T> coef, const T* __restrict__ fac,
+__global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
+                          amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_elastic_rw(Grid3 g, FactCoef<T> coef, const T* __restrict__ fac,
                                                        const uint8_t* __restrict__ mask,
                                                        const T* __restrict__ u, const T* __restrict__ b,
                                                        const DT* __restrict__ dinv, T omega, T shift,
