@@ -7,6 +7,8 @@
 // structured grid the operator is applied node by node instead (a 27-point
 // scalar stencil with the element matrix in scalar registers): one launch,
 // coalesced, no colouring, no atomics.
+#include <type_traits>
+
 #include "hexa_internal.h"
 
 namespace hexa {
@@ -31,55 +33,102 @@ __device__ constexpr int fcorner(int cx, int cy, int cz) { return 4 * cz + (cy ?
 // 3(dy+1) + 9(dz+1)) and reg (1 value; the reference's CPU filter adds
 // 1e-9 I, TopOpt.jl:75, its GPU filter nothing): A has the same coefficients wherever all 8 elements around a node
 // exist, so interior nodes take 27 FMAs; nodes on the box faces sum their
-// existing elements' rows.  A thread marches a chunk of planes along z with
-// the 3x3 neighbourhoods of planes k-1, k, k+1 in registers and those of
-// plane k+2 in flight, loading 9 values per node (neighbours in x and y come
-// from cache).
+// existing elements' rows.  A block of 64 x HB threads owns a 64 x HB tile of
+// node columns and marches a chunk of planes along z.  Each plane's tile with
+// its one-node halo (66 x (HB+2) values) is loaded once, cooperatively and
+// coalesced, into LDS as it is needed (loaded four planes ahead); a thread reads its
+// 3 x 3 neighbourhood of the plane from LDS and keeps those of planes k-1,
+// k, k+1 in registers (1.25 global loads per node and plane instead of 9).
 // With PAP (the fused filter PCG), a block also writes its partial sum of
 // x.y over the nodes it computes to pap[block] and block 0 advances the
 // iteration count; a halted state skips the launch.
+constexpr int HB = 8;                       // node rows per block (one wave each)
+constexpr int HT = 64 * HB;                 // threads per block
+constexpr int HW = 66, HR = HB + 2;         // staged tile: 66 x (HB+2) nodes
+constexpr int HTILE = HW * HR;
+static_assert(HTILE <= 2 * HT, "a plane tile is staged by at most two loads per thread");
+
 template <bool PAP>
-__global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __restrict__ Ae,
-                                                    const double* __restrict__ x, double* __restrict__ y,
-                                                    int kchunk, PcgState* st, double* __restrict__ pap) {
+__global__ __launch_bounds__(HT) void k_helm_apply(Grid3 g, const double* __restrict__ Ae,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   int kchunk, PcgState* st, double* __restrict__ pap) {
     if (PAP && st->halted) return;
+    __shared__ double tile[2][HTILE];
     const int NX = g.nx + 1, NY = g.ny + 1, NZ = g.nz + 1;
-    const int iu = blockIdx.x * 64 + threadIdx.x;
-    const int ju = blockIdx.y * 4 + threadIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y, t = tx + 64 * ty;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * HB;
+    const int iu = i0 + tx, ju = j0 + ty;
     const int k0 = blockIdx.z * kchunk, k1 = min(k0 + kchunk, NZ);
     // threads past the box edge compute a clamped node and store nothing
-    // (no early return: the PAP reduction needs every thread)
+    // (no early return: the block's barriers and the PAP reduction need
+    // every thread)
     const bool valid = iu < NX && ju < NY;
     const int i = min(iu, NX - 1), j = min(ju, NY - 1);
     const double* S = Ae + 64;
     const double reg = Ae[64 + 27];
     double xy = 0.0;
     const bool inxy = i >= 1 && i < g.nx && j >= 1 && j < g.ny;
-    // the 9 loads of a plane go through a buffer descriptor of that plane:
-    // a neighbour outside the box is an out-of-range offset and reads 0, so
-    // the loads carry no per-lane condition (a conditional load compiles to
-    // a branch and a wait of its own, which serialises the round trips)
-    auto load = [&](int kk, double (&p)[9]) {
+    // loads through a buffer descriptor of one plane: a node outside the box
+    // is an out-of-range offset and reads 0, so no load carries a per-lane
+    // condition (a conditional load compiles to a branch and a wait of its
+    // own, which serialises the round trips)
+    auto plane_rsrc = [&](int kk) {
         const bool kin = (unsigned)kk < (unsigned)NZ;
-        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<double*>(x + (kin ? (int64_t)NX * NY * kk : 0)), (short)0, kin ? NX * NY * 8 : 0, 0x00020000);
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(x + (kin ? (int64_t)NX * NY * kk : 0)),
+                                                 (short)0, kin ? NX * NY * 8 : 0, 0x00020000);
+    };
+    auto ld = [&](__amdgpu_buffer_rsrc_t r, int ii, int jj) {
+        const bool in = (unsigned)ii < (unsigned)NX && (unsigned)jj < (unsigned)NY;
+        const uint32_t off = in ? (uint32_t)(ii + NX * jj) * 8u : 0x80000000u;
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    };
+    // this thread's 3 x 3 neighbourhood of plane kk, straight from memory
+    // (the chunk's first three planes)
+    auto load9 = [&](int kk, double (&p)[9]) {
+        const auto r = plane_rsrc(kk);
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const int ii = i + dx - 1, jj = j + dy - 1;
-                const bool in = (unsigned)ii < (unsigned)NX && (unsigned)jj < (unsigned)NY;
-                const uint32_t off = in ? (uint32_t)(ii + NX * jj) * 8u : 0x80000000u;
-                p[dx + 3 * dy] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-            }
+            for (int dx = 0; dx < 3; ++dx) p[dx + 3 * dy] = ld(r, i + dx - 1, j + dy - 1);
     };
-    // Step at plane k: a, b, c hold planes k-1, k, k+1 and d receives plane
-    // k+2, in flight while plane k is computed.  Four steps rotate the four
-    // buffers back to their roles, so no register is copied across the loop
-    // (a copy of an in-flight load would make the loop wait for it).  Planes
-    // outside the box read 0 (prefetches past the last plane).
-    auto step = [&](const double (&a)[9], const double (&b)[9], const double (&c)[9], double (&d)[9], int k) {
-        load(k + 2, d);
+    // the tile of plane kk (items t and t + HT of the 66 x (HB+2) tile)
+    auto load_tile = [&](int kk, double (&v)[2]) {
+        const auto r = plane_rsrc(kk);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = t + q * HT;
+            const int ry = e / HW, rx = e - ry * HW;
+            v[q] = e < HTILE ? ld(r, i0 - 1 + rx, j0 - 1 + ry) : 0.0;
+        }
+    };
+    auto store_tile = [&](int slot, const double (&v)[2]) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (t + q * HT < HTILE) tile[slot][t + q * HT] = v[q];
+    };
+    auto read9 = [&](int slot, double (&p)[9]) {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) p[dx + 3 * dy] = tile[slot][(ty + dy) * HW + tx + dx];
+    };
+    // Step at plane k (position P of the four-step trip, r = k - k0): a, b, c
+    // hold planes k-1, k, k+1.  The tiles of planes k+2 .. k+5 are in flight
+    // in gt[(m - k0) % 4] (four planes per block, so that a CU holds enough
+    // loads in flight to cover the memory latency: one plane ahead left the
+    // stencil at 1.8 TB/s).  The step stores plane k+2's tile to LDS slot
+    // P % 2, refills its registers with plane k+6, computes plane k, and after
+    // the barrier reads plane k+2's neighbourhood into d.  Four steps rotate
+    // the four neighbourhood buffers back to their roles, so no register is
+    // copied across the loop.  One barrier per step suffices: a slot is
+    // rewritten two steps after it was read, and every thread has passed the
+    // barrier in between.
+    double gt[4][2];
+    auto step = [&](auto pos, const double (&a)[9], const double (&b)[9], const double (&c)[9], double (&d)[9],
+                    int k) {
+        constexpr int P = decltype(pos)::value, G = (P + 2) % 4;
+        store_tile(P % 2, gt[G]);
+        load_tile(k + 6, gt[G]);
         const double* w[3] = {a, b, c};
         double acc = 0.0;
         if (inxy && k >= 1 && k < g.nz) {
@@ -100,37 +149,45 @@ __global__ __launch_bounds__(256) void k_helm_apply(Grid3 g, const double* __res
             }
         }
         acc = __builtin_fma(reg, b[4], acc);
-        const bool kst = k < k1;
-        const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(
-            y + (kst ? (int64_t)NX * NY * k : 0), (short)0, kst ? NX * NY * 8 : 0, 0x00020000);
+        const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(y + (int64_t)NX * NY * k, (short)0,
+                                                                            NX * NY * 8, 0x00020000);
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, acc), ry,
                                               valid ? (uint32_t)(i + NX * j) * 8u : 0x80000000u, 0, 0);
-        if (PAP && valid && kst) xy += b[4] * acc;
+        if (PAP && valid) xy += b[4] * acc;
+        __syncthreads();
+        read9(P % 2, d);
     };
     double wa[9], wb[9], wc[9], wd[9];
-    load(k0 - 1, wa);
-    load(k0, wb);
-    load(k0 + 1, wc);
+    load9(k0 - 1, wa);
+    load9(k0, wb);
+    load9(k0 + 1, wc);
+#pragma unroll
+    for (int m = 2; m < 6; ++m) load_tile(k0 + m, gt[m % 4]);
     // (a chunk shorter than the four-step trip stops after its last plane:
     // small grids run chunks of one or two planes to fill the GPU)
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    using S3 = std::integral_constant<int, 3>;
     for (int k = k0; k < k1; k += 4) {
-        step(wa, wb, wc, wd, k);
+        step(S0{}, wa, wb, wc, wd, k);
         if (k + 1 >= k1) break;
-        step(wb, wc, wd, wa, k + 1);
+        step(S1{}, wb, wc, wd, wa, k + 1);
         if (k + 2 >= k1) break;
-        step(wc, wd, wa, wb, k + 2);
+        step(S2{}, wc, wd, wa, wb, k + 2);
         if (k + 3 >= k1) break;
-        step(wd, wa, wb, wc, k + 3);
+        step(S3{}, wd, wa, wb, wc, k + 3);
     }
     if (PAP) {
-        __shared__ double lds[4];
+        __shared__ double lds[HB];
         for (int off = 32; off > 0; off >>= 1) xy += __shfl_xor(xy, off, 64);
-        const int t = threadIdx.x + 64 * threadIdx.y;
-        if ((t & 63) == 0) lds[t >> 6] = xy;
+        if (tx == 0) lds[ty] = xy;
         __syncthreads();
         const int blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
         if (t == 0) {
-            pap[blk] = lds[0] + lds[1] + lds[2] + lds[3];
+            double sum = 0.0;
+            for (int w = 0; w < HB; ++w) sum += lds[w];
+            pap[blk] = sum;
             if (blk == 0) st->k += 1;
         }
     }
@@ -278,7 +335,7 @@ __global__ void k_helm_extract(Grid3 g, const double* __restrict__ x, double sca
 // planes per thread: 16, fewer when that leaves under 2048 blocks, more when
 // it leaves over FUSED_PAP_MAX (the fused PCG's partials)
 void helm_shape(const Grid3& g, dim3* grid, int* kchunk) {
-    const unsigned gx = (unsigned)((g.nx + 1 + 63) / 64), gy = (unsigned)((g.ny + 1 + 3) / 4);
+    const unsigned gx = (unsigned)((g.nx + 1 + 63) / 64), gy = (unsigned)((g.ny + 1 + HB - 1) / HB);
     const int NZ = g.nz + 1;
     int kc = 16;
     while (kc > 1 && (int64_t)gx * gy * ((NZ + kc - 1) / kc) < 2048) kc >>= 1;
@@ -291,7 +348,7 @@ void launch_helm_apply(hipStream_t s, const Grid3& g, const double* Ae, const do
     dim3 grid;
     int kchunk;
     helm_shape(g, &grid, &kchunk);
-    k_helm_apply<false><<<grid, dim3(64, 4, 1), 0, s>>>(g, Ae, x, y, kchunk, nullptr, nullptr);
+    k_helm_apply<false><<<grid, dim3(64, HB, 1), 0, s>>>(g, Ae, x, y, kchunk, nullptr, nullptr);
 }
 
 int launch_filter_fused_iteration(hipStream_t s, const Grid3& g, const double* Ae, PcgState* st, double* x,
@@ -302,7 +359,7 @@ int launch_filter_fused_iteration(hipStream_t s, const Grid3& g, const double* A
     const int npap = (int)(grid.x * grid.y * grid.z);
     if (npap > FUSED_PAP_MAX) return -1;
     double* part2 = partials + FUSED_PAP_MAX;
-    k_helm_apply<true><<<grid, dim3(64, 4, 1), 0, s>>>(g, Ae, p, Ap, kchunk, st, partials);
+    k_helm_apply<true><<<grid, dim3(64, HB, 1), 0, s>>>(g, Ae, p, Ap, kchunk, st, partials);
     k_fupdate<<<FB, 256, 0, s>>>(g.nn, st, partials, npap, x, p, r, Ap, dinv, part2);
     k_fcheck<<<FB, 256, 0, s>>>(g.nn, st, part2, p, r, dinv);
     return 0;
