@@ -63,6 +63,17 @@ enum { HEXA_PRECOND_JACOBI = 0, HEXA_PRECOND_MG = 1, HEXA_PRECOND_REF_MG = 2 };
 enum { HEXA_MATVEC_AUTO = 0, HEXA_MATVEC_DENSE = 1, HEXA_MATVEC_FACTORED = 2 };
 enum { HEXA_SEMANTICS_GPU = 0, HEXA_SEMANTICS_CPU = 1 };
 enum { HEXA_DIAG_EXACT = 0, HEXA_DIAG_X_ENTRY = 1 };
+/* Multigrid cycle: HEXA_CYCLE_V visits every level once per application;
+ * HEXA_CYCLE_W makes two coarse corrections on every level but the finest,
+ * each from the residual of the iterate the previous one left (levels l >= 2
+ * are visited 2^(l-1) times). */
+enum { HEXA_CYCLE_V = 0, HEXA_CYCLE_W = 1 };
+/* Default bound on the coarsest level's DOFs (its inverse is dense).  On
+ * solid/void designs the V-cycle's rediscretised coarse levels lose the
+ * structure's members; a coarsest grid of up to 2200 DOFs keeps the 60x20x20
+ * cantilever's MG-PCG at 13-24 iterations over 20 SIMP/OC steps, where 400
+ * let it grow from 17 to 49. */
+#define HEXA_COARSE_DOFS 2200
 /* Several load cases (nrhs > 1): HEXA_RHS_AUTO solves them together, each
  * kernel of a PCG iteration working on all of them in one launch, when the
  * solve is a V-cycle-preconditioned one with CPU semantics on the factored
@@ -94,7 +105,11 @@ typedef struct {
     double mg_omega;           /* Jacobi damping                            */
     int32_t nrhs;              /* load cases solved together (>= 1)        */
     int32_t diag_mode;         /* HEXA_DIAG_*                               */
-    int64_t reserved[8];
+    int32_t mg_cycle;          /* HEXA_CYCLE_*                              */
+    int32_t mg_coarse_dofs;    /* coarsen until the coarsest level has at most
+                                  this many DOFs (0 = HEXA_COARSE_DOFS);
+                                  ignored when mg_levels > 0               */
+    int64_t reserved[7];
 } hexa_options;
 
 typedef struct {

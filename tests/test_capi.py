@@ -31,6 +31,7 @@ def test_defaults_match_header_semantics():
     o = hexa.default_options()
     assert (o.E0, o.Emin, o.penal, o.nu) == (1.0, 0.0, 1.0, 0.3)
     assert o.mg_pre == 2 and o.mg_post == 2 and abs(o.mg_omega - 0.6) < 1e-15 and o.nrhs == 1
+    assert o.mg_cycle == hexa.CYCLE_V and o.mg_coarse_dofs == 0
     p = hexa.default_params()
     assert p.tol == 1e-8 and p.max_iter == 30000 and p.max_shift_attempts == 3
     assert p.shift_multiplier == 10.0 and p.precond == hexa.PRECOND_JACOBI
@@ -39,7 +40,7 @@ def test_defaults_match_header_semantics():
 
 def test_struct_sizes():
     assert C.sizeof(hexa.Grid) == 40
-    assert C.sizeof(hexa.Options) == 4 * 8 + 7 * 4 + 4 + 8 + 2 * 4 + 8 * 8
+    assert C.sizeof(hexa.Options) == 4 * 8 + 7 * 4 + 4 + 8 + 2 * 4 + 2 * 4 + 7 * 8
     assert C.sizeof(hexa.SolveReport) == 6 * 8 + 6 * 4 + 8
 
 

@@ -11,6 +11,8 @@
 * c5: a slice of the seeded serving stream from its smallest to its largest
   job, run as a mix and one by one (bitwise equal), within a memory budget
   that forces waiting; the smallest job is also run against the oracle.
+* The V-cycle on a solid/void design: c1's design after 20 OC iterations
+  (the oracle's, ``oc_c1.npz``) against c1's starting design.
 """
 import os
 import sys
@@ -55,6 +57,58 @@ def test_c1_twenty_oc_iterations_match_oracle(precision):
     assert np.abs(x - ref["x"]).max() < 1e-6
     assert np.max(np.abs(np.array(hist) - ref["compliance"]) / ref["compliance"]) < 1e-7
     assert hist[-1] < 0.6 * hist[0]
+
+
+def _c1_solve(rho_design, precision, **opt):
+    p = P.cantilever_c1()
+    with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, **opt) as s:
+        s.set_problem(p.fixed, p.f[0])
+        xp, _, _ = s.filter(rho_design, 1.5, tol=1e-13, max_iter=3000, check_every=1)
+        s.set_density(xp)
+        u, reps, st = s.solve(tol=1e-8, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2000)
+    assert st == hexa.OK and reps[0]["true_rel_res"] < 1.01e-8, reps[0]
+    return u[0], reps[0]
+
+
+def test_mg_iterations_on_the_solid_void_c1_design():
+    """After 20 OC iterations c1's filtered design has a third of its
+    elements below 0.1.  The V-cycle's iteration count there stays within
+    twice the starting design's, in both precisions, and the FP32 V-cycle
+    needs no more iterations than the FP64 one (the automatic hierarchy ends
+    on the 15x5x5 grid, 1728 DOFs)."""
+    say("mg on the c1 design after 20 OC iterations")
+    x20 = np.load(os.path.join(GOLDEN, "oc_c1.npz"))["x"]
+    x0 = P.cantilever_c1().rho
+    its = {}
+    for prec in (hexa.PREC_FP64, hexa.PREC_MIXED):
+        u0, r0 = _c1_solve(x0, prec)
+        u20, r20 = _c1_solve(x20, prec)
+        assert r0["mg_levels"] == r20["mg_levels"] == 3
+        its[prec] = (r0["iterations"], r20["iterations"])
+        if prec == hexa.PREC_FP64:
+            u_fp64 = u20
+        else:
+            assert rel(u20, u_fp64) < 1e-7
+    for prec, (smooth, design) in its.items():
+        assert design <= 2 * smooth, its
+    assert its[hexa.PREC_MIXED][1] <= 1.5 * its[hexa.PREC_FP64][1], its
+    # the 400-DOF bound (coarsest 4x2x2) needs about twice as many
+    _, r400 = _c1_solve(x20, hexa.PREC_FP64, mg_coarse_dofs=400)
+    assert r400["mg_levels"] == 5 and r400["iterations"] > 1.4 * its[hexa.PREC_FP64][1], (r400, its)
+
+
+@pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
+def test_w_cycle_solves_the_solid_void_c1_design(precision):
+    """The W-cycle (two coarse corrections per level below the finest) on
+    the 400-DOF hierarchy: the same solution as the V-cycle, in fewer
+    iterations."""
+    say("w-cycle c1 design", precision)
+    x20 = np.load(os.path.join(GOLDEN, "oc_c1.npz"))["x"]
+    uv, rv = _c1_solve(x20, precision, mg_coarse_dofs=400)
+    uw, rw = _c1_solve(x20, precision, mg_coarse_dofs=400, mg_cycle=hexa.CYCLE_W)
+    assert rw["mg_levels"] == rv["mg_levels"] == 5
+    assert rw["iterations"] < 0.8 * rv["iterations"], (rv, rw)
+    assert rel(uw, uv) < 1e-7
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "oc_c2.npz")),

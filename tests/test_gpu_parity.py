@@ -320,7 +320,9 @@ def test_mg_odd_dimensions_support_on_max_face(grid, levels, precision):
     # FP64 to 1e-12; the mixed V-cycle at its design tolerance, 1e-8
     tol, err = (1e-12, 1e-9) if precision == hexa.PREC_FP64 else (1e-8, 1e-6)
     _, _, jacobi_its, _ = oracle.pcg_cpu(grid, Ke, fac, fixed, f, tol=tol, max_iter=50000, shift_factor=1e-10)
-    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, mg_levels=levels) as s:
+    # automatic levels with a 400-DOF coarsest bound: 13x7x5 -> 7x4x3 -> 4x2x2
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, mg_levels=levels,
+                     mg_coarse_dofs=0 if levels else 400) as s:
         s.set_problem(fixed, f)
         s.set_density(rho)
         u, reps, st = s.solve(tol=tol, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU,

@@ -132,15 +132,20 @@ int32_t ensure_fac32(hexa_handle* h) {
 }
 
 // ------------------------------------------------------------------ MG
-// Number of levels: coarsen (ceil(n/2) per axis) until the coarsest grid has
-// at most COARSE_DOFS DOFs or is a single element, or to opt.mg_levels.
-constexpr int64_t COARSE_DOFS = 400;
+// Number of levels: coarsen (ceil(n/2) per axis) at least once, and until the
+// coarsest grid has at most opt.mg_coarse_dofs (default HEXA_COARSE_DOFS)
+// DOFs, or is a single element; or to opt.mg_levels.
 int choose_levels(const hexa_handle* h) {
     const int want = h->opt.mg_levels;
+    int64_t cap = h->opt.mg_coarse_dofs > 0 ? h->opt.mg_coarse_dofs : HEXA_COARSE_DOFS;
+    if (h->opt.mg_coarse_dofs == 0) {
+        const char* e = getenv("HEXA_COARSE_DOFS");   // tuning override of the default
+        if (e && atoll(e) > 0) cap = std::min<int64_t>(atoll(e), 20000);
+    }
     int nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
     int L = 1;
     auto dofs = [&] { return 3LL * (nx + 1) * (ny + 1) * (nz + 1); };
-    while ((want > 0 ? L < want : dofs() > COARSE_DOFS) && (nx > 1 || ny > 1 || nz > 1)) {
+    while ((want > 0 ? L < want : (L < 2 || dofs() > cap)) && (nx > 1 || ny > 1 || nz > 1)) {
         nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
         ++L;
     }
@@ -322,11 +327,16 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
         elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
         std::swap(xa, xb);
     }
-    elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nullptr, nb);
+    // coarse corrections: one (V-cycle), or two on every level but the
+    // finest (W-cycle), the second from the residual the first left
     Level& nx = h->levels[l + 1];
-    launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb);
-    vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x, nb);
-    launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb);
+    const int corrections = (h->opt.mg_cycle == HEXA_CYCLE_W && l > 0) ? 2 : 1;
+    for (int cc = 0; cc < corrections; ++cc) {
+        elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nullptr, nb);
+        launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb);
+        vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x, nb);
+        launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb);
+    }
     for (int it = 0; it < post; ++it) {
         elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
         std::swap(xa, xb);
@@ -1012,6 +1022,8 @@ void hexa_default_options(hexa_options* o) {
     o->mg_omega = 0.6;
     o->nrhs = 1;
     o->diag_mode = HEXA_DIAG_EXACT;
+    o->mg_cycle = HEXA_CYCLE_V;
+    o->mg_coarse_dofs = 0;
 }
 
 void hexa_default_solve_params(hexa_solve_params* p) {
@@ -1069,6 +1081,11 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
         return HEXA_E_INVALID;
     }
     if (!(o.penal > 0) || o.E0 < o.Emin) { set_error("invalid material law"); return HEXA_E_INVALID; }
+    if (o.mg_cycle != HEXA_CYCLE_V && o.mg_cycle != HEXA_CYCLE_W) { set_error("invalid mg_cycle"); return HEXA_E_INVALID; }
+    if (o.mg_coarse_dofs < 0 || o.mg_coarse_dofs > 20000) {
+        set_error("mg_coarse_dofs must be in [0, 20000]");
+        return HEXA_E_INVALID;
+    }
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) { set_error("no HIP device available"); return HEXA_E_DEVICE; }

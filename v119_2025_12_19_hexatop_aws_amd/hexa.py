@@ -35,6 +35,7 @@ MATVEC_AUTO, MATVEC_DENSE, MATVEC_FACTORED = 0, 1, 2
 SEMANTICS_GPU, SEMANTICS_CPU = 0, 1
 DIAG_EXACT, DIAG_X_ENTRY = 0, 1
 RHS_AUTO, RHS_SEQUENTIAL = 0, 1
+CYCLE_V, CYCLE_W = 0, 1
 METHOD_NAMES = {0: "Jacobi_GPU", 1: "GMG_GPU", 2: "Failed_Corrupt_Input", 3: "Jacobi_CPU", 4: "None"}
 
 
@@ -54,7 +55,8 @@ class Options(C.Structure):
                 ("ke_mode", C.c_int32), ("helm_element", C.c_int32), ("precision", C.c_int32),
                 ("matvec_kernel", C.c_int32), ("mg_levels", C.c_int32), ("mg_pre", C.c_int32),
                 ("mg_post", C.c_int32), ("mg_omega", C.c_double), ("nrhs", C.c_int32),
-                ("diag_mode", C.c_int32), ("reserved", C.c_int64 * 8)]
+                ("diag_mode", C.c_int32), ("mg_cycle", C.c_int32), ("mg_coarse_dofs", C.c_int32),
+                ("reserved", C.c_int64 * 7)]
 
 
 class SolveParams(C.Structure):
