@@ -109,7 +109,7 @@ int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke
             double* dot_part = nullptr, int nb = 1) {
     if (h->factored)
         return launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y, dot_part,
-                                        nb);
+                                        nb, h->vc_halt);
     const int64_t vs = batch_stride(3 * g.nn);
     for (int rb = 0; rb < nb; ++rb)
         launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u ? u + rb * vs : u, b ? b + rb * vs : b, dinv,
@@ -303,7 +303,7 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     hipStream_t s = h->stream;
     const int64_t nd = 3 * lv.g.nn;
     if (l == L - 1) {
-        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out, nb);
+        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out, nb, h->vc_halt);
         return;
     }
     const T* Ke = std::is_same<T, float>::value ? (const T*)lv.Ke32 : (const T*)lv.Ke64;
@@ -333,9 +333,9 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     const int corrections = (h->opt.mg_cycle == HEXA_CYCLE_W && l > 0) ? 2 : 1;
     for (int cc = 0; cc < corrections; ++cc) {
         elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nullptr, nb);
-        launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb);
+        launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb, h->vc_halt);
         vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x, nb);
-        launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb);
+        launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb, h->vc_halt);
     }
     for (int it = 0; it < post; ++it) {
         elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
@@ -464,6 +464,14 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
     // iteration count, the check schedule, the deferred x update) is decided
     // on the device, so the launches are the same every iteration.
     auto iteration = [&]() -> int32_t {
+        // the operator and the V-cycle of an iteration queued after the
+        // solve ended do nothing (every other kernel of the iteration checks
+        // the state itself)
+        struct HaltScope {
+            hexa_handle* h;
+            ~HaltScope() { h->vc_halt = nullptr; }
+        } halt_scope{h};
+        h->vc_halt = &h->d_state->halted;
         // Ap = (K + sI) p, with the per-block partials of (p, Ap)
         const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask,
                                                h->d_p, nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart);

@@ -96,6 +96,10 @@ struct hexa_handle {
     double* d_bscr = nullptr;      // batched PCG: RED_BLOCKS alpha range sums per load case
     double* d_scal = nullptr;
     hexa::PcgState* d_state = nullptr;
+    // while a PCG iteration is launched or captured: its state's halted flag,
+    // passed to the operator and V-cycle kernels so that iterations queued
+    // after the solve ended skip their work (null otherwise)
+    const int32_t* vc_halt = nullptr;
     hexa::PcgState* h_state = nullptr;   // pinned ring
 
     // preconditioner set-up of the last solve

@@ -180,7 +180,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
                                                        const DT* __restrict__ dinv, T omega, T shift,
                                                        T* __restrict__ y, int kchunk, int tiles_x, int tiles_y,
                                                        int tiles, double* __restrict__ dot_part, int nb,
-                                                       int64_t vstride, int64_t dstride) {
+                                                       int64_t vstride, int64_t dstride,
+                                                       const int32_t* __restrict__ halt) {
+    // a launch queued after its solve ended (the state's halted flag) does nothing
+    if (halt && *halt) return;
     // double-buffered by layer parity, so one barrier per layer suffices
     __shared__ T su[2][BY + 1][3][WL];   // node values of each wave's row, plane kl+2
     __shared__ T sc[2][BY][3][WL];   // each wave's results for the row above it
@@ -564,7 +567,7 @@ int64_t elastic_rw_partials(const Grid3& g) {
 template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
-                      double* dot_part, int nb) {
+                      double* dot_part, int nb, const int32_t* halt) {
     // buffer descriptors address one node plane (byte counts are 31-bit)
     if (!elastic_rw_supported(g, std::max(sizeof(T), sizeof(DT)))) {
         const int64_t vs = batch_stride(3 * g.nn);
@@ -585,19 +588,19 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
     switch (mode) {
         case OP_APPLY:
             k_elastic_rw<T, DT, OP_APPLY><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds);
+                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
             break;
         case OP_RESID:
             k_elastic_rw<T, DT, OP_RESID><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds);
+                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
             break;
         case OP_SMOOTH0:
             k_elastic_rw<T, DT, OP_SMOOTH0><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                   r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds);
+                                                                   r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
             break;
         default:
             k_elastic_rw<T, DT, OP_SMOOTH><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds);
+                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
             break;
     }
     return dot_part ? (int)elastic_rw_partials(g) : 0;
@@ -635,7 +638,7 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
             const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
             k_elastic_rw<T, T, OP_APPLY><<<tiles, dim3(WL, BY), 0, s>>>(g, coef, fac, nullptr, u, nullptr, nullptr,
                                                                         T(0), T(0), y, r.kchunk, r.tiles_x,
-                                                                        r.tiles_y, tiles, nullptr, 1, 0, 0);
+                                                                        r.tiles_y, tiles, nullptr, 1, 0, 0, nullptr);
         };
         // the GPU first reaches its sustained clocks, then every candidate is
         // timed twice (ascending, then descending) and keeps its better time
@@ -673,7 +676,7 @@ template void tune_elastic_rw<float>(hipStream_t, const Grid3&, const FactCoef<f
 #define HX_INST_RW(T, DT)                                                                            \
     template int launch_elastic_rw<T, DT>(hipStream_t, const Grid3&, OpMode, const FactCoef<T>&,     \
                                           const T*, const uint8_t*, const T*, const T*, const DT*, T, \
-                                          T, T*, double*, int);
+                                          T, T*, double*, int, const int32_t*);
 HX_INST_RW(double, double)
 HX_INST_RW(float, float)
 HX_INST_RW(float, uint16_t)
