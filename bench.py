@@ -250,14 +250,20 @@ def main():
     solves, kernels = {}, {}
     if not args.no_solve:
         u_ref = {}
-        # automatic level count (coarsest grid <= 400 DOFs) and the configured
-        # one (6 levels for 512x256x256: coarsest 16x8x8)
+        # automatic level count (coarsest grid <= 2200 DOFs) and the configured
+        # one (6 levels for 512x256x256: coarsest 16x8x8); and the mixed solve
+        # with the reference's f32-widened Ke (dense outer product, factored
+        # exact-Ke V-cycle)
         lv_cfg = int(p.meta.get("mg_levels_config", 0))
-        runs = [("mg_fp64", hexa.PREC_FP64, 0), ("mg_mixed", hexa.PREC_MIXED, 0)]
+        exact = hexa.KE_FP64
+        runs = [("mg_fp64", hexa.PREC_FP64, 0, exact), ("mg_mixed", hexa.PREC_MIXED, 0, exact)]
         if lv_cfg:
-            runs += [("mg_fp64_%dlev" % lv_cfg, hexa.PREC_FP64, lv_cfg), ("mg_mixed_%dlev" % lv_cfg, hexa.PREC_MIXED, lv_cfg)]
-        for name, prec, levels in runs:
-            s = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu, precision=prec, mg_levels=levels)
+            runs += [("mg_fp64_%dlev" % lv_cfg, hexa.PREC_FP64, lv_cfg, exact),
+                     ("mg_mixed_%dlev" % lv_cfg, hexa.PREC_MIXED, lv_cfg, exact)]
+        runs += [("mg_mixed_widened_ke", hexa.PREC_MIXED, 0, hexa.KE_F32_WIDENED)]
+        for name, prec, levels, ke in runs:
+            s = hexa.Solver(*p.grid, E0=p.E0, Emin=p.Emin, penal=p.penal, nu=p.nu, precision=prec, mg_levels=levels,
+                            ke_mode=ke)
             s.set_problem(p.fixed, p.f[0])
             s.set_density(p.rho)
             u, reps, st = s.solve(tol=1e-8, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2000)
@@ -268,7 +274,13 @@ def main():
                                 mg_levels=r["mg_levels"])
             log(name, solves[name])
             ref = "mg_fp64" if levels == 0 else "mg_fp64_%dlev" % levels
-            if prec == hexa.PREC_FP64:
+            if ke != exact:
+                # another element matrix (Float32 rounding of Ke): compliance
+                # differs from the exact-Ke solve at that level
+                c64 = solves[ref]["compliance"]
+                solves[name]["compliance_rel_diff_vs_exact_ke"] = abs(r["compliance"] - c64) / abs(c64)
+                solves[name]["solve_s_vs_exact_ke_mixed"] = solves[name]["solve_s"] / solves["mg_mixed"]["solve_s"]
+            elif prec == hexa.PREC_FP64:
                 u_ref[levels] = u[0].copy()
             else:
                 solves[name]["u_rel_err_vs_fp64"] = float(np.linalg.norm(u[0] - u_ref[levels])
@@ -276,7 +288,7 @@ def main():
                 c64 = solves[ref]["compliance"]
                 solves[name]["compliance_rel_err_vs_fp64"] = abs(r["compliance"] - c64) / abs(c64)
                 solves[name]["speedup_vs_fp64"] = solves[ref]["solve_s"] / solves[name]["solve_s"]
-            if levels == 0:
+            if levels == 0 and ke == exact:
                 # every kernel of the loop, timed on this handle's buffers
                 s.filter(p.rho, 1.5, tol=1e-3, max_iter=20, check_every=10)
                 ones = np.ones(p.n_elem)

@@ -256,6 +256,35 @@ def test_mg_preconditioner_is_symmetric_positive():
     assert np.all(Ma[~free] == 0)
 
 
+@pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
+def test_widened_ke_solve_with_the_exact_ke_v_cycle(precision):
+    """With the reference's f32-widened Ke the outer operator is the dense
+    product; the V-cycle runs the factored exact-Ke operator (the two element
+    matrices differ by Float32 rounding).  The solution is the widened-Ke
+    one: it equals the solve whose V-cycle also uses the widened Ke on the
+    dense kernel, and the oracle's widened-Ke solve to the accuracy at which
+    the library's and the oracle's Float32 element evaluations agree."""
+    say("widened Ke, exact-Ke V-cycle", precision)
+    p = P.cantilever(20, 10, 8, seed=11, smooth_passes=2)
+    kw = dict(tol=1e-11, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=500)
+    out = {}
+    for kernel in (hexa.MATVEC_AUTO, hexa.MATVEC_DENSE):
+        with hexa.Solver(*p.grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, ke_mode=hexa.KE_F32_WIDENED,
+                         matvec_kernel=kernel) as s:
+            s.set_problem(p.fixed, p.f[0])
+            s.set_density(p.rho)
+            u, reps, st = s.solve(**kw)
+        assert st == hexa.OK and reps[0]["true_rel_res"] < 1.01e-11, reps[0]
+        out[kernel] = (u[0], reps[0])
+    (ua, ra), (ud, rd) = out[hexa.MATVEC_AUTO], out[hexa.MATVEC_DENSE]
+    assert rel(ua, ud) < 1e-9
+    assert abs(ra["iterations"] - rd["iterations"]) <= 1, (ra, rd)
+    Kw = oracle.ke_hex8(1, 1, 1, 0.3, f32=True)
+    fac = oracle.stiffness_factor(p.rho, 1.0, 1e-9, 3.0)
+    uo, relo, _, _ = oracle.pcg_cpu(p.grid, Kw, fac, p.fixed, p.f[0], tol=1e-12, max_iter=50000, shift_factor=1e-10)
+    assert rel(ua, uo) < 2e-6
+
+
 def test_solve_system_gpu_mirror_and_pruning():
     """The reference-shaped entry points: generate_mesh-style node/element
     arrays, bc_indicator, pruning of voids at outer iteration >= 3."""

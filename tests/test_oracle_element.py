@@ -57,6 +57,19 @@ def test_ke_invariants():
     assert abs(u.ravel() @ K @ u.ravel() - 1 / (2 * (1 + nu))) < 1e-13
 
 
+def test_ke_f32_widened_breaks_the_rigid_body_null_space():
+    """The Float32-evaluated Ke (the reference's element matrix) no longer
+    annihilates rigid translations, and its symmetric part is indefinite on
+    the rigid-body subspace: the reason the exact Ke is the drop-in default."""
+    Kw = oracle.ke_hex8(1.0, 1.0, 1.0, 0.3, f32=True)
+    K = oracle.ke_hex8(1.0, 1.0, 1.0, 0.3)
+    R = rigid_modes()
+    assert np.abs(K @ R[:, :3]).max() < 1e-15
+    assert 1e-9 < np.abs(Kw @ R[:, :3]).max() < 1e-6
+    w = np.linalg.eigvalsh(0.5 * (Kw + Kw.T))
+    assert w[0] < -1e-9 and abs(w[0]) < 1e-7 and w[6] > 1e-2
+
+
 def test_ke_scales_with_size():
     """Ke(a h) = a Ke(h) for 3-D elasticity (used by the multigrid levels)."""
     K1 = oracle.ke_hex8(1.0, 2.0, 0.5, 0.3)

@@ -104,10 +104,10 @@ namespace {
 // (u, y); the return value is their count (0: not provided).
 // nb > 1: a batch of load cases, vectors batch_stride(3 nn) apart.
 template <typename T, typename DT>
-int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
-            const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
-            double* dot_part = nullptr, int nb = 1) {
-    if (h->factored)
+int elastic_k(hexa_handle* h, bool factored, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode,
+              const T* fac, const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
+              double* dot_part, int nb) {
+    if (factored)
         return launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y, dot_part,
                                         nb, h->vc_halt);
     const int64_t vs = batch_stride(3 * g.nn);
@@ -115,6 +115,22 @@ int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke
         launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u ? u + rb * vs : u, b ? b + rb * vs : b, dinv,
                                  omega, shift, y + rb * vs);
     return 0;
+}
+
+// The handle's operator (h->factored decides the kernel).
+template <typename T, typename DT>
+int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
+            const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
+            double* dot_part = nullptr, int nb = 1) {
+    return elastic_k<T, DT>(h, h->factored, g, coef, Ke, mode, fac, mask, u, b, dinv, omega, shift, y, dot_part, nb);
+}
+
+// The V-cycle's level operator (h->vc_factored decides the kernel).
+template <typename T, typename DT>
+int vc_elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
+               const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y, int nb = 1) {
+    return elastic_k<T, DT>(h, h->vc_factored, g, coef, Ke, mode, fac, mask, u, b, dinv, omega, shift, y, nullptr,
+                            nb);
 }
 
 int32_t ensure_fac32(hexa_handle* h) {
@@ -176,7 +192,7 @@ int32_t mg_build(hexa_handle* h) {
         lv.kscale = std::ldexp(1.0, l);
         const int64_t nd = 3 * lv.g.nn;
         double ke[576];
-        for (int i = 0; i < 576; ++i) ke[i] = lv.kscale * h->Ke[i];
+        for (int i = 0; i < 576; ++i) ke[i] = lv.kscale * h->Ke_vc[i];
         HX_TRY(dalloc(h, &lv.Ke64, 576));
         HX_CHECK(hipMemcpyAsync(lv.Ke64, ke, sizeof ke, hipMemcpyHostToDevice, h->stream));
         lv.c64 = make_fact_coef<double>(lv.kscale * h->grid.dx, lv.kscale * h->grid.dy, lv.kscale * h->grid.dz, h->opt.nu);
@@ -316,15 +332,15 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     const int pre = std::max(1, h->opt.mg_pre), post = std::max(0, h->opt.mg_post);
     // the first sweep from x = 0 is omega D^-1 b; with a second pre-sweep the
     // factored kernel does both in one pass (OP_SMOOTH0) without forming it
-    const bool fuse0 = pre >= 2 && h->factored && elastic_rw_supported(lv.g, std::max(sizeof(T), sizeof(DT)));
+    const bool fuse0 = pre >= 2 && h->vc_factored && elastic_rw_supported(lv.g, std::max(sizeof(T), sizeof(DT)));
     const int first = fuse0 ? 2 : 1;
     const int swaps = (pre - first) + post;
     T* xa = (swaps % 2 == 0) ? out : (T*)lv.xt;
     T* xb = (swaps % 2 == 0) ? (T*)lv.xt : out;
-    if (fuse0) elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH0, fac, lv.mask, nullptr, b, dinv, omega, shift, xa, nullptr, nb);
+    if (fuse0) vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH0, fac, lv.mask, nullptr, b, dinv, omega, shift, xa, nb);
     else launch_scale_dinv<T, DT>(s, nd, b, dinv, omega, xa, nb);
     for (int it = first; it < pre; ++it) {
-        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
+        vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nb);
         std::swap(xa, xb);
     }
     // coarse corrections: one (V-cycle), or two on every level but the
@@ -332,13 +348,13 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     Level& nx = h->levels[l + 1];
     const int corrections = (h->opt.mg_cycle == HEXA_CYCLE_W && l > 0) ? 2 : 1;
     for (int cc = 0; cc < corrections; ++cc) {
-        elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nullptr, nb);
+        vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nb);
         launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb, h->vc_halt);
         vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x, nb);
         launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb, h->vc_halt);
     }
     for (int it = 0; it < post; ++it) {
-        elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nullptr, nb);
+        vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nb);
         std::swap(xa, xb);
     }
 }
@@ -616,8 +632,8 @@ int32_t solve_one(hexa_handle* h, int rhs, double* u_inout, const hexa_solve_par
     if (use_mg && mg_kind == 1) HX_TRY(mg_set_shift(h, prm->shift_factor * maxd));
     // z-chunk lengths of the operator kernels, timed once per grid and kernel
     // (d_x, d_best and the level buffers are scratch at this point)
-    if (h->factored) {
-        tune_elastic_rw<double>(st, h->g, h->coef64, h->d_fac, h->d_x, h->d_best);
+    if (h->factored) tune_elastic_rw<double>(st, h->g, h->coef64, h->d_fac, h->d_x, h->d_best);
+    if (h->vc_factored) {
         if (use_mg && mg_kind == 1)
             for (size_t l = 0; l + 1 < h->levels.size(); ++l) {
                 Level& lv = h->levels[l];
@@ -1113,6 +1129,9 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
     }
     h->factored = o.matvec_kernel == HEXA_MATVEC_FACTORED ||
                   (o.matvec_kernel == HEXA_MATVEC_AUTO && o.ke_mode == HEXA_KE_FP64);
+    h->vc_factored = o.matvec_kernel != HEXA_MATVEC_DENSE;
+    if (h->vc_factored) (void)element_stiffness(grid->dx, grid->dy, grid->dz, o.nu, HEXA_KE_FP64, h->Ke_vc);
+    else std::memcpy(h->Ke_vc, h->Ke, sizeof h->Ke_vc);
     h->coef64 = make_fact_coef<double>(grid->dx, grid->dy, grid->dz, o.nu);
     h->coef32 = make_fact_coef<float>(grid->dx, grid->dy, grid->dz, o.nu);
     auto fail = [&](int32_t s) { hexa_destroy(h); return s; };
@@ -1439,14 +1458,14 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
         // it is timed on the solver's scratch vectors)
         auto sweep = [&](OpMode mode) {
             if (mx)
-                elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, mode, l0.fac32, l0.mask, (const float*)l0.x,
-                                         (const float*)l0.b, (const uint16_t*)l0.dinv, (float)omega, (float)l0.shift,
-                                         (float*)l0.xt);
+                vc_elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, mode, l0.fac32, l0.mask, (const float*)l0.x,
+                                            (const float*)l0.b, (const uint16_t*)l0.dinv, (float)omega,
+                                            (float)l0.shift, (float*)l0.xt);
             else
-                elastic<double, double>(h, g, l0.c64, l0.Ke64, mode, l0.fac64, l0.mask, h->d_best, h->d_z,
-                                        (const double*)l0.dinv, omega, l0.shift, (double*)l0.xt);
+                vc_elastic<double, double>(h, g, l0.c64, l0.Ke64, mode, l0.fac64, l0.mask, h->d_best, h->d_z,
+                                           (const double*)l0.dinv, omega, l0.shift, (double*)l0.xt);
         };
-        if (h->factored && elastic_rw_supported(g, mx ? 4 : 8))
+        if (h->vc_factored && elastic_rw_supported(g, mx ? 4 : 8))
             HX_TRY(timed(mx ? "smooth from 0, fp32/bf16" : "smooth from 0, fp64", 0, dinv + vec * 2 + fac + msk,
                          [&] { sweep(OP_SMOOTH0); }));
         HX_TRY(timed(mx ? "smooth, fp32/bf16" : "smooth, fp64", 0, dinv + vec * 3 + fac + msk,
@@ -1455,9 +1474,9 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
             char nm[40];
             std::snprintf(nm, sizeof nm, "smooth fp32/bf16, %d load cases", nb);
             HX_TRY(timed(nm, 0, dinv + vec * 3 * nb + fac + msk, [&] {
-                elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, OP_SMOOTH, l0.fac32, l0.mask, (const float*)l0.x,
-                                         (const float*)l0.b, (const uint16_t*)l0.dinv, (float)omega, (float)l0.shift,
-                                         (float*)l0.xt, nullptr, nb);
+                vc_elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, OP_SMOOTH, l0.fac32, l0.mask, (const float*)l0.x,
+                                            (const float*)l0.b, (const uint16_t*)l0.dinv, (float)omega,
+                                            (float)l0.shift, (float*)l0.xt, nb);
             }));
         }
         HX_TRY(timed(mx ? "residual, fp32" : "residual, fp64", 0, vec * 3 + fac + msk, [&] { sweep(OP_RESID); }));

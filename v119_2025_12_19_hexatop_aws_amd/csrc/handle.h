@@ -74,6 +74,13 @@ struct hexa_handle {
 
     double Ke[576];
     bool factored = false;          // exact brick Ke -> factored element kernel
+    // The V-cycle's level operators: the exact brick Ke on the factored
+    // kernel unless the dense kernel was asked for (HEXA_MATVEC_DENSE), also
+    // when the outer operator uses the dense f32-widened Ke: a preconditioner
+    // only has to approximate the operator, and the two element matrices
+    // differ by Float32 rounding (2e-6 relative).
+    bool vc_factored = false;
+    double Ke_vc[576];              // the V-cycle's element matrix at level 0
     hexa::FactCoef<double> coef64{};
     hexa::FactCoef<float> coef32{};
     double* d_Ke64 = nullptr;

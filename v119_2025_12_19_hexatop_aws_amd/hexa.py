@@ -577,7 +577,17 @@ def solve_system_gpu(nodes, elements, E, nu, bc, f, density, *, max_iter=40000, 
     pruned subset of the box given by ``config["geometry"]``.  Precision
     follows the reference's profile switch (GPUSolver.jl:187-189) extended
     with ``solver_parameters.precision``; the preconditioner follows
-    ``solver_parameters.preconditioner`` (GPUSolver.jl:231)."""
+    ``solver_parameters.preconditioner`` (GPUSolver.jl:231).
+
+    The element matrix is the exact brick Ke unless
+    ``solver_parameters.ke_mode`` is ``"f32_widened"`` (the reference's
+    Float32 Gauss-quadrature Ke widened to FP64, GPUSolver.jl:248-249).  The
+    Float32 rounding makes that matrix indefinite on the rigid-body modes
+    (eigenvalues down to -1.6e-8 against 0.064 for the smallest elastic one);
+    on a 512x256x256 cantilever it moves the compliance by 2%, and the
+    reference's own rounding pattern (its BLAS summation order) cannot be
+    reproduced, so no implementation matches its Float32 results closer than
+    that at such sizes.  The exact Ke also runs on the factored kernel."""
     config = config or {}
     sp = config.get("solver_parameters", {})
     nx, ny, nz, h = _geometry(config)
@@ -598,10 +608,12 @@ def solve_system_gpu(nodes, elements, E, nu, bc, f, density, *, max_iter=40000, 
     mg_precond = PRECOND_MG if str(sp.get("mg_mode", "reference")) == "vcycle" else PRECOND_REF_MG
     prec = PREC_MIXED if str(sp.get("precision", "fp64")) == "mixed" else PREC_FP64
     opts = dict(E0=float(E), Emin=0.0, penal=1.0, nu=float(nu), precision=prec,
-                ke_mode=KE_F32_WIDENED if str(sp.get("ke_mode", "f32_widened")) == "f32_widened" else KE_FP64,
+                ke_mode=KE_F32_WIDENED if str(sp.get("ke_mode", "fp64")) == "f32_widened" else KE_FP64,
                 diag_mode=DIAG_X_ENTRY if str(sp.get("diag_mode", "x_entry")) == "x_entry" else DIAG_EXACT,
                 mg_levels=int(sp.get("mg_levels", 0)), mg_pre=int(sp.get("mg_smoother_sweeps", 2)),
-                mg_post=int(sp.get("mg_smoother_sweeps", 2)), mg_omega=float(sp.get("mg_omega", 0.6)))
+                mg_post=int(sp.get("mg_smoother_sweeps", 2)), mg_omega=float(sp.get("mg_omega", 0.6)),
+                mg_coarse_dofs=int(sp.get("mg_coarse_dofs", 0)),
+                mg_cycle=CYCLE_W if str(sp.get("mg_cycle", "V")).upper() == "W" else CYCLE_V)
     with Solver(nx, ny, nz, h, **opts) as s:
         s.set_problem(fixed.reshape(-1), ff.reshape(-1), active if len(eid) < nE else None)
         s.set_density(rho)
