@@ -1,6 +1,9 @@
 """The row-wave operator's z-chunk length (HEXA_RW_KCHUNK fixes it; the
 library otherwise tunes it per grid by timing) changes how the planes of a
-grid are split over blocks and where the six-layer loop and its rings start.
+grid are split over blocks and where the six-layer loop and its rings start;
+its tile layout (one row of 64 lanes or two of 32 per wavefront,
+HEXA_RW_RPW fixes it, otherwise a function of the grid) how the node plane is
+split.
 It must change nothing in the results: the operator and the V-cycle are
 bitwise the same for every chunk length and agree with the oracle, and the
 PCG's (p, Ap) partial sums are formed per fixed group of planes, so a solve
@@ -47,9 +50,26 @@ def kchunk(n):
             os.environ["HEXA_RW_KCHUNK"] = old
 
 
+@contextmanager
+def layout(rpw):
+    """Fixes the row-wave tile layout (node rows per wavefront)."""
+    old = os.environ.get("HEXA_RW_RPW")
+    os.environ["HEXA_RW_RPW"] = str(rpw)
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop("HEXA_RW_RPW", None)
+        else:
+            os.environ["HEXA_RW_RPW"] = old
+
+
+@pytest.mark.parametrize("rpw", [1, 2])
 @pytest.mark.parametrize("grid", [(20, 9, 11), (70, 15, 17), (9, 20, 30)])
-def test_matvec_every_chunk_length_matches_oracle(grid):
-    say("matvec chunks", grid)
+def test_matvec_every_chunk_length_matches_oracle(grid, rpw):
+    """Both tile layouts (one node row of 64 lanes or two of 32 per
+    wavefront) give the same products, bitwise."""
+    say("matvec chunks", grid, "rows per wave", rpw)
     nx, ny, nz = grid
     rng = np.random.default_rng(21)
     rho = rng.uniform(0.001, 1.0, nx * ny * nz)
@@ -60,11 +80,14 @@ def test_matvec_every_chunk_length_matches_oracle(grid):
     dy32 = DevBuf(hip, np.zeros(x.size, np.float32))
     ys, y32s = [], []
     for kc in CHUNKS:
-        with kchunk(kc), hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0) as s:
+        with kchunk(kc), layout(rpw), hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0) as s:
             s.set_density(rho)
             ys.append(s.matvec(x))
             s.matvec_dev(dx32.p.value, dy32.p.value, hexa.PREC_MIXED, reps=1)
             y32s.append(dy32.read())
+    with layout(3 - rpw), hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0) as s:
+        s.set_density(rho)
+        y_other = s.matvec(x)
     dx32.free()
     dy32.free()
     assert np.abs(ys[0] - ref).max() <= 1e-13 * np.abs(ref).max()
@@ -72,6 +95,7 @@ def test_matvec_every_chunk_length_matches_oracle(grid):
     for kc, y, y32 in zip(CHUNKS, ys, y32s):
         assert np.array_equal(y, ys[0]), kc
         assert np.array_equal(y32, y32s[0]), kc
+    assert np.array_equal(y_other, ys[0])
 
 
 @pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
@@ -83,29 +107,33 @@ def test_vcycle_bitwise_across_chunk_lengths(precision):
     fixed, f, rho = cantilever_arrays(*grid, 5, lo=0.05)
     r = np.random.default_rng(3).standard_normal(fixed.size) * (fixed == 0)
     zs = []
-    for kc in CHUNKS:
-        with kchunk(kc), hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
-            s.set_problem(fixed, f)
-            s.set_density(rho)
-            s.solve(tol=1e-4, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=500)
-            zs.append(s.apply_precond(r))
-    assert np.linalg.norm(zs[0]) > 0
-    for kc, z in zip(CHUNKS, zs):
-        assert np.array_equal(z, zs[0]), kc
+    for rpw in (1, 2):
+        for kc in CHUNKS:
+            with kchunk(kc), layout(rpw), hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0,
+                                                      precision=precision) as s:
+                s.set_problem(fixed, f)
+                s.set_density(rho)
+                s.solve(tol=1e-4, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=500)
+                zs.append((rpw, kc, s.apply_precond(r)))
+    assert np.linalg.norm(zs[0][2]) > 0
+    for rpw, kc, z in zs:
+        assert np.array_equal(z, zs[0][2]), (rpw, kc)
 
 
+@pytest.mark.parametrize("rpw", [1, 2])
 @pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
-def test_solve_bitwise_across_chunk_lengths(precision):
+def test_solve_bitwise_across_chunk_lengths(precision, rpw):
     """The MG-PCG solve, whose alpha sums the operator kernel's (p, Ap)
     partials: the same iterations and bitwise the same u for every chunk
-    length, including the tuned one (no override)."""
-    say("solve chunks", precision)
+    length, including the tuned one (no override), in either tile layout
+    (a function of the grid unless fixed: the partials are per wavefront)."""
+    say("solve chunks", precision, "rows per wave", rpw)
     grid = (64, 24, 30)
     p = P.cantilever(*grid, seed=6, smooth_passes=2, lo=0.01)
     fixed, f, rho = p.fixed, p.f[0], p.rho
     out = []
     for kc in [0, 1, 3, 4, 8, 16, 32]:
-        with kchunk(kc), hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
+        with kchunk(kc), layout(rpw), hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision) as s:
             s.set_problem(fixed, f)
             s.set_density(rho)
             u, reps, st = s.solve(tol=1e-9, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2000)

@@ -4,17 +4,20 @@
 // difference basis: butterfly, modal diagonal + cross-component couplings,
 // transposed butterfly), mapped onto the machine differently:
 //
-//  * x runs along the 64 lanes of a wavefront.  A lane owns one node column
-//    and the element column to its +x side; neighbouring node values and the
-//    +x corner results move between lanes with DPP wave shifts, not LDS.
-//    Lanes 0 and 63 are halo: 62 owned columns per 64 loaded.
-//  * y runs across the BY waves of a block: wave w owns element row w and
-//    node row w (BY+1 node rows per tile, rows 0 and BY are halo).  The +y
-//    node row and the +y corner results are exchanged through LDS (3 values
-//    each per node and layer); the top node row is loaded by waves 0-2, one
-//    component each.  Both exchanges of a layer share one barrier: the node
-//    row staged is the one two planes ahead, and the buffers alternate with
-//    the layer parity.
+//  * x runs along the lanes: a row of LX = 64 (one row per wavefront) or 32
+//    (two rows per wavefront, one per lane half) lanes; a lane owns one node
+//    column and the element column to its +x side; neighbouring node values
+//    and the +x corner results move between lanes with DPP wave shifts, not
+//    LDS (at a half boundary the shifted-in value belongs to the other row,
+//    and only halo lanes receive it).  The first and last lane of a row are
+//    halo: LX - 2 owned columns per LX loaded.
+//  * y runs across the NR = 8 RPW rows of a block (8 waves): row r holds
+//    element row r and node row r (NR+1 node rows per tile, rows 0 and NR
+//    are halo).  The +y node row and the +y corner results are exchanged
+//    through LDS (3 values each per node and layer); the top node row is
+//    loaded by rows 0-2, one component each.  Both exchanges of a layer
+//    share one barrier: the node row staged is the one two planes ahead, and
+//    the buffers alternate with the layer parity.
 //  * z is marched, with the node planes and element factors of the next three
 //    layers in flight (two in the first smoothing sweep).  The butterfly is separable, so the x-y part of a
 //    node plane's transform is computed once and used by the element layers
@@ -44,10 +47,18 @@ namespace {
 #ifndef HEXA_RW_BY
 #define HEXA_RW_BY 8
 #endif
-constexpr int WL = 64;           // lanes per wavefront = node columns per tile
-constexpr int OWN_X = WL - 2;    // owned node columns per tile
-constexpr int BY = HEXA_RW_BY;   // wavefronts per block = node rows per tile
-constexpr int OWN_Y = BY - 1;    // owned node rows per tile
+constexpr int WL = 64;           // lanes per wavefront
+constexpr int BY = HEXA_RW_BY;   // wavefronts per block
+// Tile layouts: RPW node rows per wavefront, LX = 64 / RPW lanes (node
+// columns) per row, NR = BY RPW rows per tile; the first and last column
+// and the first row are halo.
+template <int RPW>
+struct RwTile {
+    static constexpr int LX = WL / RPW;
+    static constexpr int OWN_X = LX - 2;
+    static constexpr int NR = BY * RPW;
+    static constexpr int OWN_Y = NR - 1;
+};
 constexpr int NXCD = 8;
 constexpr int DOT_GROUP = 4;     // output planes per (u, y) partial sum
 // Cache policy of the output stores: non-temporal (the gfx950 "nt" bit),
@@ -172,7 +183,7 @@ __device__ __forceinline__ T ld1_dinv(Rsrc r, uint32_t off, const uint16_t*) {
 // 134-138 VGPRs and ran one block per CU (0.547 ms at 512x256x256, 0.449
 // held to 128; its BF16-diagonal form spills 12 bytes per lane).  The FP64
 // code is the same as with the default bound.
-template <typename T, typename DT, int MODE>
+template <typename T, typename DT, int MODE, int RPW>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
                           amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_elastic_rw(Grid3 g, FactCoef<T> coef, const T* __restrict__ fac,
                                                        const uint8_t* __restrict__ mask,
@@ -184,11 +195,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
                                                        const int32_t* __restrict__ halt) {
     // a launch queued after its solve ended (the state's halted flag) does nothing
     if (halt && *halt) return;
+    constexpr int LX = RwTile<RPW>::LX, OWN_X = RwTile<RPW>::OWN_X, NR = RwTile<RPW>::NR,
+                  OWN_Y = RwTile<RPW>::OWN_Y;
     // double-buffered by layer parity, so one barrier per layer suffices
-    __shared__ T su[2][BY + 1][3][WL];   // node values of each wave's row, plane kl+2
-    __shared__ T sc[2][BY][3][WL];   // each wave's results for the row above it
+    __shared__ T su[2][NR + 1][3][LX];   // node values of each row, plane kl+2
+    __shared__ T sc[2][NR][3][LX];       // each row's results for the row above it
 
+    // lane l of wave w: node column lx of tile row ro (RPW rows per wave)
     const int l = threadIdx.x, w = threadIdx.y;
+    const int lx = l % LX, ro = w * RPW + l / LX;
     T cl[3][2], cm[3][2];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -219,13 +234,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
         if (dot_part) dot_part += rb * dstride;
     }
 
-    const int i = tx * OWN_X - 1 + l;
-    const int j = ty * OWN_Y - 1 + w;
+    const int i = tx * OWN_X - 1 + lx;
+    const int j = ty * OWN_Y - 1 + ro;
     const int kb = tz * kchunk;
     const int ke = min(kb + kchunk, g.nz + 1);
     const bool col_in = (unsigned)i <= (unsigned)g.nx && (unsigned)j <= (unsigned)g.ny;
-    const bool elem = l < WL - 1 && (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-    const bool owner = l >= 1 && l <= WL - 2 && w >= 1 && col_in;
+    const bool elem = lx < LX - 1 && (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const bool owner = lx >= 1 && lx <= LX - 2 && ro >= 1 && col_in;
     const int64_t plane_n = (int64_t)(g.nx + 1) * (g.ny + 1);
     const int64_t plane_e = (int64_t)g.nx * g.ny;
 
@@ -237,11 +252,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
     const uint32_t moff = owner ? (uint32_t)nrow : OOB;
     const uint32_t doff = owner ? (uint32_t)(nrow * 3 * sizeof(DT)) : OOB;
     const int64_t vplane = 3 * plane_n;   // vector entries per node plane
-    // the tile's top node row (BY waves own element rows 0..BY-1 and node
-    // rows 1..BY-1): waves 0-2 load component w of its nodes as an extra
-    const int jx = ty * OWN_Y - 1 + BY;
-    const bool x_in = w < 3 && (unsigned)i <= (unsigned)g.nx && (unsigned)jx <= (unsigned)g.ny;
-    const int64_t xent = (i + (int64_t)(g.nx + 1) * jx) * 3 + w;
+    // the tile's top node row (rows 0..NR-1 hold element rows 0..NR-1 and
+    // node rows 1..NR-1 are owned): rows 0-2 load component ro of its nodes
+    // as an extra
+    const int jx = ty * OWN_Y - 1 + NR;
+    const bool x_in = ro < 3 && (unsigned)i <= (unsigned)g.nx && (unsigned)jx <= (unsigned)g.ny;
+    const int64_t xent = (i + (int64_t)(g.nx + 1) * jx) * 3 + ro;
     const uint32_t xoff = x_in ? (uint32_t)(xent * sizeof(T)) : OOB;
     const uint32_t xdoff = x_in ? (uint32_t)(xent * sizeof(DT)) : OOB;
 
@@ -279,8 +295,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
     auto stage = [&](auto par, const T (&o)[3], const T x) {
         constexpr int P = decltype(par)::value;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) su[P][w][c][l] = o[c];
-        if (w < 3) su[P][BY][w][l] = x;
+        for (int c = 0; c < 3; ++c) su[P][ro][c][lx] = o[c];
+        if (ro < 3) su[P][NR][ro][lx] = x;
     };
     auto load_fac = [&](int lay) -> T {
         const bool in = (unsigned)lay < (unsigned)g.nz;
@@ -387,15 +403,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
                 N1[c] = c01 + from_prev(c11);
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) sc[P][w][c][l] = N1[c];
+            for (int c = 0; c < 3; ++c) sc[P][ro][c][lx] = N1[c];
         }
         stage(par, unx, xnx);
         __syncthreads();
         T o[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            upnx[c] = su[P][w + 1][c][l];
-            const T below = sc[P][w > 0 ? w - 1 : 0][c][l];
+            upnx[c] = su[P][ro + 1][c][lx];
+            const T below = sc[P][ro > 0 ? ro - 1 : 0][c][lx];
             const T Au = fmad(shift, ulo[c], N0[c] + (owner ? below : T(0)));
             T v;
             if (MODE == OP_APPLY) v = Au;
@@ -428,8 +444,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
         T up0[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            up0[c] = su[0][w + 1][c][l];
-            up[0][c] = su[1][w + 1][c][l];
+            up0[c] = su[0][ro + 1][c][lx];
+            up[0][c] = su[1][ro + 1][c][lx];
         }
         plane_modes(uu[0], up0, PP[0]);
         __syncthreads();   // the staging buffers are rewritten by the first layer
@@ -511,7 +527,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
 
 namespace {
 struct RwShape {
-    int tiles_x, tiles_y, tiles_z, kchunk;
+    int rpw, tiles_x, tiles_y, tiles_z, kchunk;
 };
 
 // Chunk lengths (node planes per block along z) measured fastest for a grid
@@ -519,10 +535,12 @@ struct RwShape {
 // not tuned use the block-count rule below.
 enum TuneKind { TK_RW64 = 0, TK_RW32 = 1 };
 std::mutex g_tune_mu;
-std::map<std::array<int, 4>, int> g_tuned;
+std::map<std::array<int, 5>, int> g_tuned;
+int rw_rpw(const Grid3& g, int kind);
 int tuned_chunk(const Grid3& g, int kind) {
+    const int rpw = rw_rpw(g, kind);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    const auto it = g_tuned.find({g.nx, g.ny, g.nz, kind});
+    const auto it = g_tuned.find({g.nx, g.ny, g.nz, kind, rpw});
     return it == g_tuned.end() ? 0 : it->second;
 }
 
@@ -536,10 +554,36 @@ int chunk_override() {
     return v >= 1 && v <= 64 ? v : 0;
 }
 
+// Rows per wavefront for grid g, a function of the grid alone (the (u, y)
+// partial sums are per wavefront, so the layout sets their summation order):
+// the layout whose tiles touch fewer 128-byte lines of the node planes per
+// plane (one line more than the row's bytes per loaded row, for the
+// unaligned start), the one-row layout unless the two-row one saves 3%.
+// Two rows of 32 lanes load 17 rows per 15 owned instead of 9 per 7, but
+// touch more partial lines per byte and split the x axis into 30-node
+// instead of 62-node tiles: at 256x128x128 (257 node columns: 5 tiles of 62
+// hold 310) the FP64 product took 0.065 ms against 0.079, at 512x256x256
+// 0.489 against 0.476.  HEXA_RW_RPW=1|2 fixes it (tests).
+template <int RPW>
+double rw_lines(const Grid3& g, int esize) {
+    using TL = RwTile<RPW>;
+    const double tx = (g.nx + 1 + TL::OWN_X - 1) / TL::OWN_X, ty = (g.ny + 1 + TL::OWN_Y - 1) / TL::OWN_Y;
+    return tx * ty * (TL::NR + 1) * (double)((TL::LX * 3 * esize + 127) / 128 + 1);
+}
+int rw_rpw(const Grid3& g, int kind) {
+    const char* e = getenv("HEXA_RW_RPW");
+    if (e && (e[0] == '1' || e[0] == '2') && !e[1]) return e[0] - '0';
+    const int es = kind == TK_RW64 ? 8 : 4;
+    return rw_lines<2>(g, es) < 0.97 * rw_lines<1>(g, es) ? 2 : 1;
+}
+
 RwShape rw_shape(const Grid3& g, int kind = TK_RW64, int kchunk = 0) {
     RwShape r;
-    r.tiles_x = (g.nx + 1 + OWN_X - 1) / OWN_X;
-    r.tiles_y = (g.ny + 1 + OWN_Y - 1) / OWN_Y;
+    r.rpw = rw_rpw(g, kind);
+    const int own_x = r.rpw == 2 ? RwTile<2>::OWN_X : RwTile<1>::OWN_X;
+    const int own_y = r.rpw == 2 ? RwTile<2>::OWN_Y : RwTile<1>::OWN_Y;
+    r.tiles_x = (g.nx + 1 + own_x - 1) / own_x;
+    r.tiles_y = (g.ny + 1 + own_y - 1) / own_y;
     // node planes per block: enough blocks to fill the chip, long enough
     // chunks to amortise the one recomputed element layer per block
     if (!kchunk) kchunk = chunk_override();
@@ -558,7 +602,8 @@ bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
     return !fallback_kernels_forced() && 3 * (int64_t)(g.nx + 1) * (g.ny + 1) * (int64_t)elem_bytes <= INT32_MAX;
 }
 
-// One partial per (tile column, wave, group of DOT_GROUP output planes).
+// One partial per (tile column, wave, group of DOT_GROUP output planes) of
+// the FP64 kernel (the only one that forms them).
 int64_t elastic_rw_partials(const Grid3& g) {
     const RwShape r = rw_shape(g, TK_RW64, DOT_GROUP);
     return (int64_t)r.tiles_x * r.tiles_y * BY * ((g.nz + 1 + DOT_GROUP - 1) / DOT_GROUP);
@@ -585,23 +630,20 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
     const int64_t vs = batch_stride(3 * g.nn), ds = elastic_rw_partials(g);
     dim3 block(WL, BY);
     const unsigned nblk = (unsigned)tiles * (unsigned)nb;
+    auto go = [&](auto md, auto rp) {
+        constexpr int M = decltype(md)::value, RP = decltype(rp)::value;
+        k_elastic_rw<T, DT, M, RP><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y, r.kchunk,
+                                                         r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
+    };
+    auto by_layout = [&](auto md) {
+        if (r.rpw == 2) go(md, std::integral_constant<int, 2>{});
+        else go(md, std::integral_constant<int, 1>{});
+    };
     switch (mode) {
-        case OP_APPLY:
-            k_elastic_rw<T, DT, OP_APPLY><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
-            break;
-        case OP_RESID:
-            k_elastic_rw<T, DT, OP_RESID><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                 r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
-            break;
-        case OP_SMOOTH0:
-            k_elastic_rw<T, DT, OP_SMOOTH0><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                   r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
-            break;
-        default:
-            k_elastic_rw<T, DT, OP_SMOOTH><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y,
-                                                                  r.kchunk, r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
-            break;
+        case OP_APPLY: by_layout(std::integral_constant<int, OP_APPLY>{}); break;
+        case OP_RESID: by_layout(std::integral_constant<int, OP_RESID>{}); break;
+        case OP_SMOOTH0: by_layout(std::integral_constant<int, OP_SMOOTH0>{}); break;
+        default: by_layout(std::integral_constant<int, OP_SMOOTH>{}); break;
     }
     return dot_part ? (int)elastic_rw_partials(g) : 0;
 }
@@ -646,9 +688,14 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         auto launch = [&](int kc) {
             const RwShape r = rw_shape(g, kind, kc);
             const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
-            k_elastic_rw<T, T, OP_APPLY><<<tiles, dim3(WL, BY), 0, s>>>(g, coef, fac, nullptr, u, nullptr, nullptr,
-                                                                        T(0), T(0), y, r.kchunk, r.tiles_x,
-                                                                        r.tiles_y, tiles, nullptr, 1, 0, 0, nullptr);
+            if (r.rpw == 2)
+                k_elastic_rw<T, T, OP_APPLY, 2><<<tiles, dim3(WL, BY), 0, s>>>(
+                    g, coef, fac, nullptr, u, nullptr, nullptr, T(0), T(0), y, r.kchunk, r.tiles_x, r.tiles_y, tiles,
+                    nullptr, 1, 0, 0, nullptr);
+            else
+                k_elastic_rw<T, T, OP_APPLY, 1><<<tiles, dim3(WL, BY), 0, s>>>(
+                    g, coef, fac, nullptr, u, nullptr, nullptr, T(0), T(0), y, r.kchunk, r.tiles_x, r.tiles_y, tiles,
+                    nullptr, 1, 0, 0, nullptr);
         };
         // the GPU first reaches its sustained clocks, then every candidate is
         // timed twice (ascending, then descending) and keeps its better time
@@ -671,7 +718,7 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         best = ok && best ? cand[best - 1] : 0;
         if (best) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            g_tuned[{g.nx, g.ny, g.nz, kind}] = best;
+            g_tuned[{g.nx, g.ny, g.nz, kind, rw_rpw(g, kind)}] = best;
         }
     }
     DeviceOpGuard lock;
