@@ -109,7 +109,7 @@ int elastic_k(hexa_handle* h, bool factored, const Grid3& g, const FactCoef<T>& 
               double* dot_part, int nb) {
     if (factored)
         return launch_elastic_rw<T, DT>(h->stream, g, mode, coef, fac, mask, u, b, dinv, omega, shift, y, dot_part,
-                                        nb, h->vc_halt);
+                                        nb, h->vc_halt, h->vc_halt_stride);
     const int64_t vs = batch_stride(3 * g.nn);
     for (int rb = 0; rb < nb; ++rb)
         launch_elastic_op<T, DT>(h->stream, g, mode, Ke, fac, mask, u ? u + rb * vs : u, b ? b + rb * vs : b, dinv,
@@ -319,7 +319,7 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     hipStream_t s = h->stream;
     const int64_t nd = 3 * lv.g.nn;
     if (l == L - 1) {
-        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out, nb, h->vc_halt);
+        launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out, nb, h->vc_halt, h->vc_halt_stride);
         return;
     }
     const T* Ke = std::is_same<T, float>::value ? (const T*)lv.Ke32 : (const T*)lv.Ke64;
@@ -349,9 +349,10 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
     const int corrections = (h->opt.mg_cycle == HEXA_CYCLE_W && l > 0) ? 2 : 1;
     for (int cc = 0; cc < corrections; ++cc) {
         vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_RESID, fac, lv.mask, xa, b, dinv, omega, shift, (T*)lv.r, nb);
-        launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb, h->vc_halt);
+        launch_restrict<T>(s, lv.g, nx.g, (const T*)lv.r, nx.mask, (T*)nx.b, nb, h->vc_halt, h->vc_halt_stride);
         vcycle_level<T, DT>(h, l + 1, (const T*)nx.b, (T*)nx.x, nb);
-        launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb, h->vc_halt);
+        launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb, h->vc_halt,
+                              h->vc_halt_stride);
     }
     for (int it = 0; it < post; ++it) {
         vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nb);
@@ -437,6 +438,22 @@ struct AttemptResult {
     double final_rel = 0.0, best_rel = INFINITY;
 };
 
+// While a PCG iteration is launched or captured: the operator and V-cycle
+// kernels read the halted flags of the nb load cases' states, so that work
+// queued after a case's solve ended returns at once (every other kernel of
+// the iteration checks the state itself).
+struct HaltScope {
+    hexa_handle* h;
+    HaltScope(hexa_handle* hh, int nb) : h(hh) {
+        h->vc_halt = &h->d_state[0].halted;
+        h->vc_halt_stride = nb > 1 ? (int)(sizeof(PcgState) / sizeof(int32_t)) : 0;
+    }
+    ~HaltScope() {
+        h->vc_halt = nullptr;
+        h->vc_halt_stride = 0;
+    }
+};
+
 // One PCG attempt with shift s from the current d_x.  Returns the final
 // device state (synchronised).
 // pk: 0 Jacobi, 1 multigrid V-cycle, 2 reference two-level preconditioner.
@@ -483,11 +500,7 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
         // the operator and the V-cycle of an iteration queued after the
         // solve ended do nothing (every other kernel of the iteration checks
         // the state itself)
-        struct HaltScope {
-            hexa_handle* h;
-            ~HaltScope() { h->vc_halt = nullptr; }
-        } halt_scope{h};
-        h->vc_halt = &h->d_state->halted;
+        HaltScope halt_scope(h, 1);
         // Ap = (K + sI) p, with the per-block partials of (p, Ap)
         const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask,
                                                h->d_p, nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart);
@@ -816,6 +829,8 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
     float* z32 = mx ? (float*)h->levels[0].x : nullptr;
     const int64_t pstride = elastic_rw_partials(h->g);
     auto iteration = [&]() -> int32_t {
+        // a load case's operator and V-cycle work stops once its solve ended
+        HaltScope halt_scope(h, nb);
         const int nq = elastic<double, double>(h, h->g, h->coef64, h->d_Ke64, OP_APPLY, h->d_fac, h->d_mask, h->d_p,
                                                nullptr, nullptr, 0.0, s, h->d_Ap, h->d_dotpart, nb);
         if (nq <= 0) {

@@ -107,6 +107,7 @@ struct hexa_handle {
     // passed to the operator and V-cycle kernels so that iterations queued
     // after the solve ended skip their work (null otherwise)
     const int32_t* vc_halt = nullptr;
+    int vc_halt_stride = 0;         // int32 entries between the flags of a batch's load cases
     hexa::PcgState* h_state = nullptr;   // pinned ring
 
     // preconditioner set-up of the last solve

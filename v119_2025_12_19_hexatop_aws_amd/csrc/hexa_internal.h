@@ -154,7 +154,7 @@ void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactC
 template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
-                      double* dot_part = nullptr, int nb = 1, const int32_t* halt = nullptr);
+                      double* dot_part = nullptr, int nb = 1, const int32_t* halt = nullptr, int hs = 0);
 // Whether the row-wave kernel takes grid g (its buffer descriptors address one
 // node plane of elem_bytes-sized values with 31-bit byte counts); otherwise
 // launch_elastic_rw falls back to the tile kernel, which has no OP_SMOOTH0.
@@ -182,15 +182,15 @@ void launch_coarsen_mask(hipStream_t s, const Grid3& fine, const Grid3& coarse, 
 
 // Multigrid transfers (k_mg.hip)
 // (nb > 1: a batch of right-hand sides, vectors batch_stride(3 nn) apart on
-// each level).  halt, when set, points to a solve's halted flag: a launch
-// queued after the solve ended returns at once (the V-cycle kernels and the
-// iteration's operator take it).
+// each level).  halt, when set, points to a solve's halted flag (of load
+// case rb at halt[rb * hs]): a launch queued after the solve ended returns at
+// once (the V-cycle kernels and the iteration's operator take it).
 template <typename T>
 void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
-                     const uint8_t* cmask, T* rc, int nb = 1, const int32_t* halt = nullptr);
+                     const uint8_t* cmask, T* rc, int nb = 1, const int32_t* halt = nullptr, int hs = 0);
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
-                        const uint8_t* fmask, T* xf, int nb = 1, const int32_t* halt = nullptr);
+                        const uint8_t* fmask, T* xf, int nb = 1, const int32_t* halt = nullptr, int hs = 0);
 // active[e] = (float)rho[e] > thr; RED_BLOCKS per-block counts in partials
 void launch_active_from_rho(hipStream_t s, int64_t ne, const double* rho, float thr, uint8_t* active,
                             double* partials);
@@ -206,7 +206,7 @@ inline int64_t gj_col_doubles(int64_t n) { return GJ_PANEL * n; }
 void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, double* colbuf);
 template <typename T>
 void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y, int nb = 1,
-                       const int32_t* halt = nullptr);
+                       const int32_t* halt = nullptr, int hs = 0);
 
 // Vectors and reductions (k_vector.hip)
 template <typename T>

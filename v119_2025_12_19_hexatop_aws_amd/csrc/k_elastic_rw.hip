@@ -192,9 +192,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
                                                        T* __restrict__ y, int kchunk, int tiles_x, int tiles_y,
                                                        int tiles, double* __restrict__ dot_part, int nb,
                                                        int64_t vstride, int64_t dstride,
-                                                       const int32_t* __restrict__ halt) {
-    // a launch queued after its solve ended (the state's halted flag) does nothing
-    if (halt && *halt) return;
+                                                       const int32_t* __restrict__ halt, int hs) {
     constexpr int LX = RwTile<RPW>::LX, OWN_X = RwTile<RPW>::OWN_X, NR = RwTile<RPW>::NR,
                   OWN_Y = RwTile<RPW>::OWN_Y;
     // double-buffered by layer parity, so one barrier per layer suffices
@@ -226,6 +224,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
     const int xcd = L % NXCD, q = total / NXCD, r = total % NXCD;
     const int tb = xcd * q + min(xcd, r) + L / NXCD;
     const int rb = tb / tiles, t = tb % tiles;
+    // a launch queued after its solve (of right-hand side rb) ended does nothing
+    if (halt && halt[rb * hs]) return;
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
     if (rb) {
         if (u) u += rb * vstride;
@@ -612,7 +612,7 @@ int64_t elastic_rw_partials(const Grid3& g) {
 template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
-                      double* dot_part, int nb, const int32_t* halt) {
+                      double* dot_part, int nb, const int32_t* halt, int hs) {
     // buffer descriptors address one node plane (byte counts are 31-bit)
     if (!elastic_rw_supported(g, std::max(sizeof(T), sizeof(DT)))) {
         const int64_t vs = batch_stride(3 * g.nn);
@@ -633,7 +633,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
     auto go = [&](auto md, auto rp) {
         constexpr int M = decltype(md)::value, RP = decltype(rp)::value;
         k_elastic_rw<T, DT, M, RP><<<nblk, block, 0, s>>>(g, coef, fac, mask, u, b, dinv, omega, shift, y, r.kchunk,
-                                                         r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt);
+                                                         r.tiles_x, r.tiles_y, tiles, dot_part, nb, vs, ds, halt, hs);
     };
     auto by_layout = [&](auto md) {
         if (r.rpw == 2) go(md, std::integral_constant<int, 2>{});
@@ -691,11 +691,11 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
             if (r.rpw == 2)
                 k_elastic_rw<T, T, OP_APPLY, 2><<<tiles, dim3(WL, BY), 0, s>>>(
                     g, coef, fac, nullptr, u, nullptr, nullptr, T(0), T(0), y, r.kchunk, r.tiles_x, r.tiles_y, tiles,
-                    nullptr, 1, 0, 0, nullptr);
+                    nullptr, 1, 0, 0, nullptr, 0);
             else
                 k_elastic_rw<T, T, OP_APPLY, 1><<<tiles, dim3(WL, BY), 0, s>>>(
                     g, coef, fac, nullptr, u, nullptr, nullptr, T(0), T(0), y, r.kchunk, r.tiles_x, r.tiles_y, tiles,
-                    nullptr, 1, 0, 0, nullptr);
+                    nullptr, 1, 0, 0, nullptr, 0);
         };
         // the GPU first reaches its sustained clocks, then every candidate is
         // timed twice (ascending, then descending) and keeps its better time
@@ -733,7 +733,7 @@ template void tune_elastic_rw<float>(hipStream_t, const Grid3&, const FactCoef<f
 #define HX_INST_RW(T, DT)                                                                            \
     template int launch_elastic_rw<T, DT>(hipStream_t, const Grid3&, OpMode, const FactCoef<T>&,     \
                                           const T*, const uint8_t*, const T*, const T*, const DT*, T, \
-                                          T, T*, double*, int, const int32_t*);
+                                          T, T*, double*, int, const int32_t*, int);
 HX_INST_RW(double, double)
 HX_INST_RW(float, float)
 HX_INST_RW(float, uint16_t)

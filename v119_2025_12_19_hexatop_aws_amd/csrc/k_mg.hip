@@ -85,8 +85,8 @@ template <typename T>
 __global__ __launch_bounds__(TB) void k_restrict(Grid3 f, Grid3 c, FastDiv dnx, FastDiv dny, uint32_t ncn,
                                                  const T* __restrict__ rf, const uint8_t* __restrict__ cmask,
                                                  T* __restrict__ rc, int64_t fs, int64_t cs,
-                                                 const int32_t* __restrict__ halt) {
-    if (halt && *halt) return;   // the solve this V-cycle belongs to has ended
+                                                 const int32_t* __restrict__ halt, int hs) {
+    if (halt && halt[blockIdx.y * hs]) return;   // the solve this V-cycle belongs to has ended
     rf += blockIdx.y * fs;   // right-hand side blockIdx.y of a batch
     rc += blockIdx.y * cs;
     const uint32_t n = blockIdx.x * TB + threadIdx.x;
@@ -130,8 +130,8 @@ template <typename T>
 __global__ __launch_bounds__(TB) void k_prolong_add(Grid3 f, Grid3 c, FastDiv dpr, FastDiv dny, uint32_t npairs,
                                                     const T* __restrict__ xc, const uint8_t* __restrict__ fmask,
                                                     T* __restrict__ xf, int64_t fs, int64_t cs,
-                                                    const int32_t* __restrict__ halt) {
-    if (halt && *halt) return;
+                                                    const int32_t* __restrict__ halt, int hs) {
+    if (halt && halt[blockIdx.y * hs]) return;
     xc += blockIdx.y * cs;   // right-hand side blockIdx.y of a batch
     xf += blockIdx.y * fs;
     const uint32_t q = blockIdx.x * TB + threadIdx.x;
@@ -390,8 +390,8 @@ __global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t p0, int n
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __restrict__ Ainv,
                                                     const T* __restrict__ x, T* __restrict__ y, int64_t vs,
-                                                    const int32_t* __restrict__ halt) {
-    if (halt && *halt) return;
+                                                    const int32_t* __restrict__ halt, int hs) {
+    if (halt && halt[blockIdx.y * hs]) return;
     x += blockIdx.y * vs;   // right-hand side blockIdx.y of a batch
     y += blockIdx.y * vs;
     const int lane = threadIdx.x & 63;
@@ -408,22 +408,22 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
 
 template <typename T>
 void launch_restrict(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* rf,
-                     const uint8_t* cmask, T* rc, int nb, const int32_t* halt) {
+                     const uint8_t* cmask, T* rc, int nb, const int32_t* halt, int hs) {
     const uint32_t ncn = (uint32_t)coarse.nn;
     k_restrict<T><<<dim3((ncn + TB - 1) / TB, nb), TB, 0, s>>>(fine, coarse, fast_div(coarse.nx + 1),
                                                                fast_div(coarse.ny + 1), ncn, rf, cmask, rc,
                                                                batch_stride(3 * fine.nn), batch_stride(3 * coarse.nn),
-                                                               halt);
+                                                               halt, hs);
 }
 
 template <typename T>
 void launch_prolong_add(hipStream_t s, const Grid3& fine, const Grid3& coarse, const T* xc,
-                        const uint8_t* fmask, T* xf, int nb, const int32_t* halt) {
+                        const uint8_t* fmask, T* xf, int nb, const int32_t* halt, int hs) {
     const uint32_t ppr = (uint32_t)(fine.nx + 2) / 2;   // node pairs per fine row
     const uint32_t npairs = ppr * (uint32_t)((fine.ny + 1) * (fine.nz + 1));
     k_prolong_add<T><<<dim3((npairs + TB - 1) / TB, nb), TB, 0, s>>>(fine, coarse, fast_div(ppr), fast_div(fine.ny + 1),
                                                                      npairs, xc, fmask, xf, batch_stride(3 * fine.nn),
-                                                                     batch_stride(3 * coarse.nn), halt);
+                                                                     batch_stride(3 * coarse.nn), halt, hs);
 }
 
 void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, const uint8_t* fmask,
@@ -450,19 +450,20 @@ void launch_gauss_jordan(hipStream_t s, int64_t n, double* A, double* rowbuf, do
 }
 
 template <typename T>
-void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y, int nb, const int32_t* halt) {
-    k_dense_gemv<T><<<dim3((unsigned)((n + 3) / 4), nb), 256, 0, s>>>(n, Ainv, x, y, batch_stride(n), halt);
+void launch_dense_gemv(hipStream_t s, int64_t n, const double* Ainv, const T* x, T* y, int nb, const int32_t* halt,
+                       int hs) {
+    k_dense_gemv<T><<<dim3((unsigned)((n + 3) / 4), nb), 256, 0, s>>>(n, Ainv, x, y, batch_stride(n), halt, hs);
 }
 
 template void launch_restrict<double>(hipStream_t, const Grid3&, const Grid3&, const double*,
-                                      const uint8_t*, double*, int, const int32_t*);
+                                      const uint8_t*, double*, int, const int32_t*, int);
 template void launch_restrict<float>(hipStream_t, const Grid3&, const Grid3&, const float*,
-                                     const uint8_t*, float*, int, const int32_t*);
+                                     const uint8_t*, float*, int, const int32_t*, int);
 template void launch_prolong_add<double>(hipStream_t, const Grid3&, const Grid3&, const double*,
-                                         const uint8_t*, double*, int, const int32_t*);
+                                         const uint8_t*, double*, int, const int32_t*, int);
 template void launch_prolong_add<float>(hipStream_t, const Grid3&, const Grid3&, const float*,
-                                        const uint8_t*, float*, int, const int32_t*);
-template void launch_dense_gemv<double>(hipStream_t, int64_t, const double*, const double*, double*, int, const int32_t*);
-template void launch_dense_gemv<float>(hipStream_t, int64_t, const double*, const float*, float*, int, const int32_t*);
+                                        const uint8_t*, float*, int, const int32_t*, int);
+template void launch_dense_gemv<double>(hipStream_t, int64_t, const double*, const double*, double*, int, const int32_t*, int);
+template void launch_dense_gemv<float>(hipStream_t, int64_t, const double*, const float*, float*, int, const int32_t*, int);
 
 }  // namespace hexa
