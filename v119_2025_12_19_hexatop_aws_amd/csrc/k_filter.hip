@@ -38,15 +38,19 @@ __device__ constexpr int fcorner(int cx, int cy, int cz) { return 4 * cz + (cy ?
 // its one-node halo (66 x (HB+2) values) is loaded once, cooperatively and
 // coalesced, into LDS as it is needed (loaded four planes ahead); a thread reads its
 // 3 x 3 neighbourhood of the plane from LDS and keeps those of planes k-1,
-// k, k+1 in registers (1.25 global loads per node and plane instead of 9).
+// k, k+1 in registers (1.5 global loads per node and plane instead of 9).
 // With PAP (the fused filter PCG), a block also writes its partial sum of
 // x.y over the nodes it computes to pap[block] and block 0 advances the
 // iteration count; a halted state skips the launch.
-constexpr int HB = 8;                       // node rows per block (one wave each)
+// Node rows per block (one wave each).  Four-wave blocks (four blocks per
+// CU) beat eight: at 512x256x256 0.248 against 0.287 ms per apply, at 144^3
+// 0.053 against 0.059 ms (one and two waves per block were slower: their
+// tiles carry more halo rows per owned row).
+constexpr int HB = 4;
 constexpr int HT = 64 * HB;                 // threads per block
 constexpr int HW = 66, HR = HB + 2;         // staged tile: 66 x (HB+2) nodes
 constexpr int HTILE = HW * HR;
-static_assert(HTILE <= 2 * HT, "a plane tile is staged by at most two loads per thread");
+constexpr int NQ = (HTILE + HT - 1) / HT;   // tile values staged per thread
 
 template <bool PAP>
 __global__ __launch_bounds__(HT) void k_helm_apply(Grid3 g, const double* __restrict__ Ae,
@@ -91,19 +95,19 @@ __global__ __launch_bounds__(HT) void k_helm_apply(Grid3 g, const double* __rest
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) p[dx + 3 * dy] = ld(r, i + dx - 1, j + dy - 1);
     };
-    // the tile of plane kk (items t and t + HT of the 66 x (HB+2) tile)
-    auto load_tile = [&](int kk, double (&v)[2]) {
+    // the tile of plane kk (items t, t + HT, ... of the 66 x (HB+2) tile)
+    auto load_tile = [&](int kk, double (&v)[NQ]) {
         const auto r = plane_rsrc(kk);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const int e = t + q * HT;
             const int ry = e / HW, rx = e - ry * HW;
             v[q] = e < HTILE ? ld(r, i0 - 1 + rx, j0 - 1 + ry) : 0.0;
         }
     };
-    auto store_tile = [&](int slot, const double (&v)[2]) {
+    auto store_tile = [&](int slot, const double (&v)[NQ]) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < NQ; ++q)
             if (t + q * HT < HTILE) tile[slot][t + q * HT] = v[q];
     };
     auto read9 = [&](int slot, double (&p)[9]) {
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(HT) void k_helm_apply(Grid3 g, const double* __rest
     // copied across the loop.  One barrier per step suffices: a slot is
     // rewritten two steps after it was read, and every thread has passed the
     // barrier in between.
-    double gt[4][2];
+    double gt[4][NQ];
     auto step = [&](auto pos, const double (&a)[9], const double (&b)[9], const double (&c)[9], double (&d)[9],
                     int k) {
         constexpr int P = decltype(pos)::value, G = (P + 2) % 4;
