@@ -50,3 +50,18 @@ def test_invalid_arguments_are_rejected_without_a_device():
     st = hexa.lib().hexa_create(C.byref(bad), None, C.byref(h))
     assert st == hexa.E_INVALID and "invalid grid" in hexa.last_error()
     assert hexa.lib().hexa_solve(None, None, None, None) == hexa.E_INVALID
+
+
+def test_estimate_counts_the_coarsest_inverse():
+    """hexa_estimate_bytes follows the multigrid level rule: a 64^3 grid's
+    coarsest level is 8^3 (2187 DOFs) under the default 2200-DOF bound and
+    4^3 (375 DOFs) under a 400-DOF bound; the dense inverse and its 32-column
+    factorisation panels make the difference (host-only call)."""
+    g = hexa.Grid(64, 64, 64, 1.0, 1.0, 1.0)
+    est = hexa.lib().hexa_estimate_bytes
+    d = est(C.byref(g), C.byref(hexa.default_options()))
+    s = est(C.byref(g), C.byref(hexa.default_options(mg_coarse_dofs=400)))
+    expect = (2187 ** 2 - 375 ** 2) * 8 + 3 * 32 * (2187 - 375) * 8
+    assert abs((d - s) - expect) <= 1, (d - s, expect)
+    # explicit levels: 64 -> 32 -> 16 -> 8 -> 4 (5 levels) is the 400-DOF case
+    assert est(C.byref(g), C.byref(hexa.default_options(mg_levels=5))) == s

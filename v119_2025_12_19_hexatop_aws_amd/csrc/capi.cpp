@@ -148,25 +148,34 @@ int32_t ensure_fac32(hexa_handle* h) {
 }
 
 // ------------------------------------------------------------------ MG
-// Number of levels: coarsen (ceil(n/2) per axis) at least once, and until the
-// coarsest grid has at most opt.mg_coarse_dofs (default HEXA_COARSE_DOFS)
-// DOFs, or is a single element; or to opt.mg_levels.
-int choose_levels(const hexa_handle* h) {
-    const int want = h->opt.mg_levels;
-    int64_t cap = h->opt.mg_coarse_dofs > 0 ? h->opt.mg_coarse_dofs : HEXA_COARSE_DOFS;
-    if (h->opt.mg_coarse_dofs == 0) {
+}  // namespace
+
+// Number of multigrid levels for an nx x ny x nz grid: coarsen (ceil(n/2) per
+// axis) at least once, and until the coarsest grid has at most
+// opt.mg_coarse_dofs (default HEXA_COARSE_DOFS) DOFs, or is a single element;
+// or to opt.mg_levels.  The coarsest level's DOF count goes to *coarse_dofs.
+namespace hexa {
+int mg_level_count(int nx, int ny, int nz, const hexa_options& opt, int64_t* coarse_dofs) {
+    const int want = opt.mg_levels;
+    int64_t cap = opt.mg_coarse_dofs > 0 ? opt.mg_coarse_dofs : HEXA_COARSE_DOFS;
+    if (opt.mg_coarse_dofs == 0) {
         const char* e = getenv("HEXA_COARSE_DOFS");   // tuning override of the default
         if (e && atoll(e) > 0) cap = std::min<int64_t>(atoll(e), 20000);
     }
-    int nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
     int L = 1;
     auto dofs = [&] { return 3LL * (nx + 1) * (ny + 1) * (nz + 1); };
     while ((want > 0 ? L < want : (L < 2 || dofs() > cap)) && (nx > 1 || ny > 1 || nz > 1)) {
         nx = (nx + 1) / 2; ny = (ny + 1) / 2; nz = (nz + 1) / 2;
         ++L;
     }
+    if (coarse_dofs) *coarse_dofs = dofs();
     return L;
 }
+}  // namespace hexa
+
+namespace {
+
+int choose_levels(const hexa_handle* h) { return mg_level_count(h->g.nx, h->g.ny, h->g.nz, h->opt, nullptr); }
 
 bool mixed(const hexa_handle* h) { return h->opt.precision == HEXA_PREC_MIXED; }
 

@@ -198,6 +198,9 @@ void launch_refmg_dcinv(hipStream_t s, const Grid3& fine, const Grid3& coarse, c
                         const double* kd8, double* dcinv);
 void launch_coarse_assemble(hipStream_t s, const Grid3& g, const double* Ke, const double* fac,
                             const uint8_t* mask, double shift, double* A);
+// Multigrid level count of an nx x ny x nz grid under opt, and the coarsest
+// level's DOF count (capi.cpp).
+int mg_level_count(int nx, int ny, int nz, const hexa_options& opt, int64_t* coarse_dofs);
 // In-place inverse of the SPD n x n matrix A; rowbuf holds gj_row_doubles(n)
 // and colbuf gj_col_doubles(n) values.
 constexpr int GJ_PANEL = 32;

@@ -74,7 +74,13 @@ extern "C" int64_t hexa_estimate_bytes(const hexa_grid* g, const hexa_options* o
     b += nn * lvl * 8.0 / 7.0 + ne * (8 + 4 + 8) * 8.0 / 7.0;
     // filter nodal vectors and element work arrays
     b += nn * 6 * 8 + ne * 12 * 8;
-    b += 1200.0 * 1200.0 * 8 + (64 << 20);   // coarse inverse bound + small buffers
+    // the coarsest level's dense inverse and its factorisation panels, and
+    // small buffers
+    hexa_options d;
+    if (!o) hexa_default_options(&d);
+    int64_t nc = 0;
+    hexa::mg_level_count(g->nx, g->ny, g->nz, o ? *o : d, &nc);
+    b += (double)nc * nc * 8 + 3.0 * hexa::GJ_PANEL * nc * 8 + (64 << 20);
     return (int64_t)b;
 }
 
