@@ -637,7 +637,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
     }
     RwShape r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32);
     // partial sums need chunks that start on group boundaries: multiples of
-    // DOT_GROUP planes (every candidate length is a power of two)
+    // DOT_GROUP planes (every tuned length of 4 or more is one)
     if (dot_part && r.kchunk % DOT_GROUP) r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32,
                                                       ((r.kchunk + DOT_GROUP - 1) / DOT_GROUP) * DOT_GROUP);
     const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
@@ -692,13 +692,38 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         const RwShape r4 = rw_shape(g, kind, 4), r32 = rw_shape(g, kind, 32);
         const bool small = (int64_t)r4.tiles_x * r4.tiles_y * r4.tiles_z < 4096;
         const bool large = (int64_t)r32.tiles_x * r32.tiles_y * r32.tiles_z >= 2048;
-        int cand[16], NC = 0;
-        for (int c : {32, 16, 8, 4}) cand[NC++] = c;
+        int cand[24], NC = 0;
+        auto add = [&](int c) {
+            for (int q = 0; q < NC; ++q)
+                if (cand[q] == c) return;
+            cand[NC++] = c;
+        };
+        for (int c : {32, 16, 8, 4}) add(c);
         if (small)
-            for (int c : {2, 1}) cand[NC++] = c;
+            for (int c : {2, 1}) add(c);
         if (large)
-            for (int c : {20, 24, 28, 36, 40, 44, 48}) cand[NC++] = c;
-        float t[16] = {};
+            for (int c : {20, 24, 28, 36, 40, 44, 48}) add(c);
+        // Lengths that fit the grid into 1-4 full rounds of the blocks the
+        // chip holds at once (one FP64 or two FP32 blocks per CU), rounded up
+        // to whole groups of partial-sum planes: at 256x128x128 one round
+        // of 44-plane chunks (243 blocks) took 0.055 ms against 0.064 ms
+        // with the best power of two.
+        {
+            int dev = 0, ncu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+                ncu = 256;
+            const int64_t cap = (int64_t)ncu * (sizeof(T) == 8 ? 1 : 2);
+            const int64_t txy = (int64_t)r4.tiles_x * r4.tiles_y;
+            for (int rounds = 1; rounds <= 4; ++rounds) {
+                const int64_t nch = rounds * cap / txy;
+                if (nch < 1) continue;
+                int kc = (int)((g.nz + 1 + nch - 1) / nch);
+                kc = (kc + DOT_GROUP - 1) / DOT_GROUP * DOT_GROUP;
+                if (kc >= 4 && kc <= 64) add(kc);
+            }
+        }
+        float t[24] = {};
         auto launch = [&](int kc) {
             const RwShape r = rw_shape(g, kind, kc);
             const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
