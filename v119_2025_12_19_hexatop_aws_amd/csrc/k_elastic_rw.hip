@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <array>
 #include <climits>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -755,6 +756,13 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         for (int c = 0; ok && c < NC; ++c)
             if (!best || t[c] < t[best - 1]) best = c + 1;
         best = ok && best ? cand[best - 1] : 0;
+        // HEXA_TUNE_LOG=1: each tuned grid's candidate times and choice on stderr
+        if (const char* lg = getenv("HEXA_TUNE_LOG"); lg && lg[0] == '1') {
+            fprintf(stderr, "[hexa tune] %dx%dx%d %s layout %d:", g.nx, g.ny, g.nz, sizeof(T) == 8 ? "fp64" : "fp32",
+                    rw_rpw(g, kind));
+            for (int c = 0; c < NC; ++c) fprintf(stderr, " %d:%.4f", cand[c], t[c] / 3.0f);
+            fprintf(stderr, " -> %d\n", best);
+        }
         if (best) {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tuned[{g.nx, g.ny, g.nz, kind, rw_rpw(g, kind)}] = best;
