@@ -128,8 +128,9 @@ int elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke
 // The V-cycle's level operator (h->vc_factored decides the kernel).
 template <typename T, typename DT>
 int vc_elastic(hexa_handle* h, const Grid3& g, const FactCoef<T>& coef, const T* Ke, OpMode mode, const T* fac,
-               const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y, int nb = 1) {
-    return elastic_k<T, DT>(h, h->vc_factored, g, coef, Ke, mode, fac, mask, u, b, dinv, omega, shift, y, nullptr,
+               const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y, int nb = 1,
+               double* dot_part = nullptr) {
+    return elastic_k<T, DT>(h, h->vc_factored, g, coef, Ke, mode, fac, mask, u, b, dinv, omega, shift, y, dot_part,
                             nb);
 }
 
@@ -321,15 +322,18 @@ int32_t mg_set_shift(hexa_handle* h, double s) {
 
 // nb > 1: a batch of load cases (b, out and the level vectors hold nb
 // vectors, batch_stride(3 nn) apart on each level).
+// With dots, the last smoothing sweep also writes partial sums of
+// (b, out) there; the return value is their count (0: not written, e.g.
+// with no post-smoothing sweep).
 template <typename T, typename DT>
-void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
+int vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1, double* dots = nullptr) {
     Level& lv = h->levels[l];
     const int L = (int)h->levels.size();
     hipStream_t s = h->stream;
     const int64_t nd = 3 * lv.g.nn;
     if (l == L - 1) {
         launch_dense_gemv<T>(s, h->n_coarse, h->d_Ainv, b, out, nb, h->vc_halt, h->vc_halt_stride);
-        return;
+        return 0;
     }
     const T* Ke = std::is_same<T, float>::value ? (const T*)lv.Ke32 : (const T*)lv.Ke64;
     const T* fac = std::is_same<T, float>::value ? (const T*)lv.fac32 : (const T*)lv.fac64;
@@ -363,17 +367,21 @@ void vcycle_level(hexa_handle* h, int l, const T* b, T* out, int nb = 1) {
         launch_prolong_add<T>(s, lv.g, nx.g, (const T*)nx.x, lv.mask, xa, nb, h->vc_halt,
                               h->vc_halt_stride);
     }
+    int nq = 0;
     for (int it = 0; it < post; ++it) {
-        vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nb);
+        nq = vc_elastic<T, DT>(h, lv.g, coef, Ke, OP_SMOOTH, fac, lv.mask, xa, b, dinv, omega, shift, xb, nb,
+                               it == post - 1 ? dots : nullptr);
         std::swap(xa, xb);
     }
+    return nq;
 }
 
 // z = M^-1 r (FP64 in and out); nb load cases batch_stride(3 nn) apart.
-void apply_mg(hexa_handle* h, const double* r, double* z, int nb = 1) {
+// dots: as vcycle_level (FP64 V-cycle only; the return value is the count).
+int apply_mg(hexa_handle* h, const double* r, double* z, int nb = 1, double* dots = nullptr) {
     if (h->levels.size() == 1) {
         launch_dense_gemv<double>(h->stream, h->n_coarse, h->d_Ainv, r, z, nb);
-        return;
+        return 0;
     }
     if (mixed(h)) {
         Level& l0 = h->levels[0];
@@ -383,9 +391,9 @@ void apply_mg(hexa_handle* h, const double* r, double* z, int nb = 1) {
         vcycle_level<float, uint16_t>(h, 0, (const float*)l0.b, (float*)l0.x, nb);
         for (int rb = 0; rb < nb; ++rb)
             launch_convert<float, double>(h->stream, nd, (const float*)l0.x + rb * vs, z + rb * vs);
-    } else {
-        vcycle_level<double, double>(h, 0, r, z, nb);
+        return 0;
     }
+    return vcycle_level<double, double>(h, 0, r, z, nb, dots);
 }
 
 
@@ -531,15 +539,28 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
                          use_mg ? 0 : 1);
         // (copies only when the check improved the best residual)
         if (!cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
+        // beta: the V-cycle's last smoothing sweep forms the (r, z) partials
+        // (with the FP32 residual it smooths, in the mixed V-cycle) and beta
+        // is the standard (r, z)_new / (r, z); without such a sweep, the
+        // flexible beta from a separate pass over r, z and Ap
+        double* const bscr = h->d_partials + (MAX_RED_Q - 1) * RED_BLOCKS;
         if (mx) {
-            vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32);
-            launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_partials);
-            launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            const int nz = vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32, 1, h->d_dotpart);
+            if (nz > 0) {
+                launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, bscr);
+            } else {
+                launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_partials);
+                launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            }
             launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, defer_x ? h->d_x : nullptr);
         } else if (use_mg) {
-            apply_mg(h, h->d_r, h->d_z);
-            launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);
-            launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            const int nz = apply_mg(h, h->d_r, h->d_z, 1, h->d_dotpart);
+            if (nz > 0) {
+                launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, bscr);
+            } else {
+                launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_partials);
+                launch_pcg_flex_beta(st, h->d_state, h->d_partials);
+            }
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr);
         } else if (pk == 2) {
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr);
@@ -837,6 +858,8 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
     float* r32 = mx ? (float*)h->levels[0].b : nullptr;
     float* z32 = mx ? (float*)h->levels[0].x : nullptr;
     const int64_t pstride = elastic_rw_partials(h->g);
+    // the V-cycle's (r, z) partials (from its FP32 or FP64 level-0 kernels)
+    const int64_t vstride = elastic_rw_partials(h->g, mx ? sizeof(float) : sizeof(double));
     auto iteration = [&]() -> int32_t {
         // a load case's operator and V-cycle work stops once its solve ended
         HaltScope halt_scope(h, nb);
@@ -851,15 +874,25 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
         launch_pcg_update(st, nd, h->d_state, mx ? nullptr : h->d_x, h->d_p, h->d_r, h->d_Ap, nullptr, h->d_bpart,
                           r32, nb, vs, PS);
         launch_pcg_check(st, h->d_state, h->d_bpart, 1, 0, 1, 0, nb, PS);
+        // beta as in the single solve: from the V-cycle's last smoothing
+        // sweep's (r, z) partials where it forms them
         if (mx) {
-            vcycle_level<float, uint16_t>(h, 0, r32, z32, nb);
-            launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_bpart, nb, vs, PS);
-            launch_pcg_flex_beta(st, h->d_state, h->d_bpart, nb, PS);
+            const int nz = vcycle_level<float, uint16_t>(h, 0, r32, z32, nb, h->d_dotpart);
+            if (nz > 0) {
+                launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, h->d_bscr, nb, vstride);
+            } else {
+                launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_r, z32, h->d_Ap, h->d_bpart, nb, vs, PS);
+                launch_pcg_flex_beta(st, h->d_state, h->d_bpart, nb, PS);
+            }
             launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, h->d_x, nb, vs);
         } else {
-            apply_mg(h, h->d_r, h->d_z, nb);
-            launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_bpart, nb, vs, PS);
-            launch_pcg_flex_beta(st, h->d_state, h->d_bpart, nb, PS);
+            const int nz = apply_mg(h, h->d_r, h->d_z, nb, h->d_dotpart);
+            if (nz > 0) {
+                launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, h->d_bscr, nb, vstride);
+            } else {
+                launch_pcg_flex_dots(st, nd, h->d_state, h->d_r, h->d_z, h->d_Ap, h->d_bpart, nb, vs, PS);
+                launch_pcg_flex_beta(st, h->d_state, h->d_bpart, nb, PS);
+            }
             launch_pcg_p_update(st, nd, h->d_state, h->d_p, h->d_z, nullptr, nullptr, nb, vs);
         }
         return check_launch();
@@ -1022,7 +1055,11 @@ int32_t ensure_solver_vectors(hexa_handle* h) {
     HX_TRY(dalloc(h, &h->d_b, nv));
     HX_TRY(dalloc(h, &h->d_diag, nd));
     HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
-    HX_TRY(dalloc(h, &h->d_dotpart, (size_t)std::max<int64_t>(1, elastic_rw_partials(h->g)) * nb));
+    // (p, Ap) partials of the FP64 operator and (r, z) partials of the
+    // V-cycle's FP32 or FP64 last sweep, which may tile the grid differently
+    HX_TRY(dalloc(h, &h->d_dotpart,
+                  (size_t)std::max<int64_t>({1, elastic_rw_partials(h->g, sizeof(double)),
+                                             elastic_rw_partials(h->g, sizeof(float))}) * nb));
     HX_TRY(dalloc(h, &h->d_scal, 16 + (size_t)nb));
     HX_TRY(dalloc(h, &h->d_state, (size_t)nb));
     if (nb > 1) {
@@ -1494,6 +1531,22 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
                          [&] { sweep(OP_SMOOTH0); }));
         HX_TRY(timed(mx ? "smooth, fp32/bf16" : "smooth, fp64", 0, dinv + vec * 3 + fac + msk,
                      [&] { sweep(OP_SMOOTH); }));
+        if (h->vc_factored && elastic_rw_supported(g, mx ? 4 : 8)) {
+            // the V-cycle's last sweep, which also forms the (r, z) partials of beta
+            const double parts = 8.0 * (double)elastic_rw_partials(g, mx ? sizeof(float) : sizeof(double));
+            HX_TRY(timed(mx ? "last smooth + (r,z) partials, fp32/bf16" : "last smooth + (r,z) partials, fp64", 0,
+                         dinv + vec * 3 + fac + msk + parts, [&] {
+                             if (mx)
+                                 vc_elastic<float, uint16_t>(h, g, l0.c32, l0.Ke32, OP_SMOOTH, l0.fac32, l0.mask,
+                                                             (const float*)l0.x, (const float*)l0.b,
+                                                             (const uint16_t*)l0.dinv, (float)omega, (float)l0.shift,
+                                                             (float*)l0.xt, 1, h->d_dotpart);
+                             else
+                                 vc_elastic<double, double>(h, g, l0.c64, l0.Ke64, OP_SMOOTH, l0.fac64, l0.mask,
+                                                            h->d_best, h->d_z, (const double*)l0.dinv, omega, l0.shift,
+                                                            (double*)l0.xt, 1, h->d_dotpart);
+                         }));
+        }
         if (nb > 1 && mx) {
             char nm[40];
             std::snprintf(nm, sizeof nm, "smooth fp32/bf16, %d load cases", nb);
@@ -1523,7 +1576,7 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
         HX_TRY(timed("pcg update (r, r32)", 0, 28.0 * nd, [&] {
             launch_pcg_update(st, nd, h->d_state, nullptr, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials, r32);
         }));
-        HX_TRY(timed("pcg flexible dots (z32)", 0, 20.0 * nd, [&] {
+        HX_TRY(timed("pcg flex dots z32 (no last sweep)", 0, 20.0 * nd, [&] {
             launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_z, z32, h->d_Ap, h->d_partials);
         }));
         HX_TRY(timed("pcg p update + deferred x (z32)", 0, 36.0 * nd, [&] {
@@ -1533,7 +1586,7 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
         HX_TRY(timed("pcg update (x, r)", 0, 48.0 * nd, [&] {
             launch_pcg_update(st, nd, h->d_state, h->d_best, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials);
         }));
-        HX_TRY(timed("pcg flexible dots", 0, 24.0 * nd, [&] {
+        HX_TRY(timed("pcg flexible dots (no last sweep)", 0, 24.0 * nd, [&] {
             launch_pcg_flex_dots(st, nd, h->d_state, h->d_z, h->d_best, h->d_Ap, h->d_partials);
         }));
         HX_TRY(timed("pcg p update", 0, 24.0 * nd, [&] {

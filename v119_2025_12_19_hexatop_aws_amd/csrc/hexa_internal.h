@@ -145,12 +145,13 @@ void launch_elastic_mfma(hipStream_t s, const Grid3& g, OpMode mode, const T* Ke
 template <typename T, typename DT>
 void launch_elastic_fact(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                          const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y);
-// Returns the number of (u, y) partial sums written to dot_part (0 when
-// dot_part is null or the kernel cannot provide them); their values, and so
-// their fixed-order sum, do not depend on the kernel's z-chunk length.
+// Returns the number of partial sums written to dot_part (0 when dot_part
+// is null or the kernel cannot provide them): of (u, y) for OP_APPLY, of
+// (b, y) for OP_SMOOTH; their values, and so their fixed-order sum, do not
+// depend on the kernel's z-chunk length.
 // nb > 1: nb right-hand sides in one launch, vectors batch_stride(3 nn)
-// apart and partial sums elastic_rw_partials(g) apart; fac, mask and dinv
-// are shared.
+// apart and partial sums elastic_rw_partials(g, sizeof(T)) apart; fac, mask
+// and dinv are shared.
 template <typename T, typename DT>
 int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef<T>& coef, const T* fac,
                       const uint8_t* mask, const T* u, const T* b, const DT* dinv, T omega, T shift, T* y,
@@ -159,8 +160,10 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
 // node plane of elem_bytes-sized values with 31-bit byte counts); otherwise
 // launch_elastic_rw falls back to the tile kernel, which has no OP_SMOOTH0.
 bool elastic_rw_supported(const Grid3& g, size_t elem_bytes);
-// Number of partial sums launch_elastic_rw writes for grid g.
-int64_t elastic_rw_partials(const Grid3& g);
+// Number of partial sums launch_elastic_rw writes for grid g with
+// elem_bytes-sized values (the FP64 and FP32 kernels may tile a grid
+// differently).
+int64_t elastic_rw_partials(const Grid3& g, size_t elem_bytes = 8);
 // Picks the z-chunk length of the row-wave kernels for grid g by timing
 // (process-wide, once per grid and kernel); u is read, y overwritten.
 template <typename T>
@@ -314,6 +317,10 @@ void launch_pcg_p_update(hipStream_t s, int64_t n, const PcgState* st, double* p
 // order (through RED_BLOCKS range sums in scratch when n > RED_BLOCKS).
 void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, int breakdown_check,
                         double* scratch, int nb = 1, int64_t pstride = 0);
+// beta = (r, z)_new / (r, z) from the n (r, z) partials of the V-cycle's last
+// smoothing sweep (fixed order, as launch_pcg_alpha_n), and rz = (r, z)_new.
+void launch_pcg_beta_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, double* scratch,
+                       int nb = 1, int64_t pstride = 0);
 void launch_pcg_flex_dots_z32(hipStream_t s, int64_t n, const PcgState* st, const double* r, const float* z,
                               const double* Ap, double* partials, int nb = 1, int64_t vs = 0, int ps = 0);
 void launch_pcg_p_update_z32(hipStream_t s, int64_t n, const PcgState* st, double* p, const float* z,

@@ -435,8 +435,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WL * BY),
             o[c] = ((ot.mb >> c) & 1) ? T(0) : v;
         }
         st3(ot.ry, ooff, o);
-        if (dot_part && ot.has_out && owner)
-            acc += fmad((double)ulo[2], (double)o[2], fmad((double)ulo[1], (double)o[1], (double)ulo[0] * o[0]));
+        // partial sums of (u, y) for the operator, of (b, y) for a smoothing
+        // sweep (the preconditioned residual's (r, z) when it is the V-cycle's
+        // last)
+        if (dot_part && ot.has_out && owner) {
+            const T(&dl)[3] = MODE == OP_APPLY ? ulo : ot.bv;
+            acc += fmad((double)dl[2], (double)o[2], fmad((double)dl[1], (double)o[1], (double)dl[0] * o[0]));
+        }
     };
 
     // Rings indexed by layer: with the chunk's layers counted from 0 at kb-1,
@@ -619,8 +624,8 @@ bool elastic_rw_supported(const Grid3& g, size_t elem_bytes) {
 
 // One partial per (tile column, wave, group of DOT_GROUP output planes) of
 // the FP64 kernel (the only one that forms them).
-int64_t elastic_rw_partials(const Grid3& g) {
-    const RwShape r = rw_shape(g, TK_RW64, DOT_GROUP);
+int64_t elastic_rw_partials(const Grid3& g, size_t elem_bytes) {
+    const RwShape r = rw_shape(g, elem_bytes == 8 ? TK_RW64 : TK_RW32, DOT_GROUP);
     return (int64_t)r.tiles_x * r.tiles_y * BY * ((g.nz + 1 + DOT_GROUP - 1) / DOT_GROUP);
 }
 
@@ -642,7 +647,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
     if (dot_part && r.kchunk % DOT_GROUP) r = rw_shape(g, sizeof(T) == 8 ? TK_RW64 : TK_RW32,
                                                       ((r.kchunk + DOT_GROUP - 1) / DOT_GROUP) * DOT_GROUP);
     const int tiles = r.tiles_x * r.tiles_y * r.tiles_z;
-    const int64_t vs = batch_stride(3 * g.nn), ds = elastic_rw_partials(g);
+    const int64_t vs = batch_stride(3 * g.nn), ds = elastic_rw_partials(g, sizeof(T));
     dim3 block(WL, BY);
     const unsigned nblk = (unsigned)tiles * (unsigned)nb;
     auto go = [&](auto md, auto rp) {
@@ -660,7 +665,7 @@ int launch_elastic_rw(hipStream_t s, const Grid3& g, OpMode mode, const FactCoef
         case OP_SMOOTH0: by_layout(std::integral_constant<int, OP_SMOOTH0>{}); break;
         default: by_layout(std::integral_constant<int, OP_SMOOTH>{}); break;
     }
-    return dot_part ? (int)elastic_rw_partials(g) : 0;
+    return dot_part ? (int)ds : 0;
 }
 
 // Times K(rho)u (APPLY) for each candidate chunk length on this grid and

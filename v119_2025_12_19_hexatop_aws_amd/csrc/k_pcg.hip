@@ -158,6 +158,30 @@ __global__ __launch_bounds__(RED_THREADS) void k_alpha_n(PcgState* st, const dou
     }
 }
 
+// The standard (Fletcher-Reeves) beta = (r, z)_new / (r, z) from n partials
+// of (r, z) written by the V-cycle's last smoothing sweep; k_beta takes the
+// RED_BLOCKS range sums of k_sum_ranges instead.
+__device__ __forceinline__ void set_beta(PcgState* st, double rz) {
+    st->beta = rz / st->rz;
+    st->rz = rz;
+}
+__global__ __launch_bounds__(RED_THREADS) void k_beta_n(PcgState* st, const double* partials, int n, int64_t pin) {
+    st += blockIdx.y;
+    partials += blockIdx.y * pin;
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double rz = final_sum_n(partials, n, lds);
+    if (threadIdx.x == 0) set_beta(st, rz);
+}
+__global__ __launch_bounds__(RED_THREADS) void k_beta(PcgState* st, const double* partials, int ps) {
+    st += blockIdx.y;
+    partials += (int64_t)blockIdx.y * ps;
+    if (st->halted) return;
+    __shared__ double lds[RED_THREADS / 64];
+    const double rz = final_sum(partials, 0, lds);
+    if (threadIdx.x == 0) set_beta(st, rz);
+}
+
 // x += alpha p; r -= alpha Ap; partials: (r,r) and, when minv is given,
 // (r, minv r) for the Jacobi preconditioner.  r32, when given, receives the
 // new residual in FP32 (the mixed V-cycle's input).  With x null the x update
@@ -436,6 +460,15 @@ void launch_pcg_alpha_n(hipStream_t s, PcgState* st, const double* partials, int
     } else {
         k_sum_ranges<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(st, partials, n, scratch, pstride, RED_BLOCKS);
         k_alpha<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, scratch, breakdown_check, RED_BLOCKS);
+    }
+}
+void launch_pcg_beta_n(hipStream_t s, PcgState* st, const double* partials, int64_t n, double* scratch, int nb,
+                       int64_t pstride) {
+    if (n <= RED_BLOCKS) {
+        k_beta_n<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, partials, (int)n, pstride);
+    } else {
+        k_sum_ranges<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(st, partials, n, scratch, pstride, RED_BLOCKS);
+        k_beta<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, scratch, RED_BLOCKS);
     }
 }
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
