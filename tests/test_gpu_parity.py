@@ -405,6 +405,31 @@ def test_multi_rhs_load_cases(precision):
     assert [r["iterations"] for r in reps_s] == [r["iterations"] for r in reps]
 
 
+def test_filter_before_batched_solve():
+    """A handle of several load cases whose first device work is a density
+    filter: the buffers that the filter, the element kernels and the solver
+    share are sized for the batch whichever allocates them first, and the
+    batched solve then equals the one on a fresh handle bitwise."""
+    say("filter before batched solve")
+    nx, ny, nz, nrhs = 12, 6, 6, 3
+    p = P.cantilever(nx, ny, nz, seed=4, smooth_passes=1, lo=0.3)
+    f = P.point_loads(nx, ny, nz, nrhs, seed=4)
+    kw = dict(tol=1e-10, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, max_iter=2000)
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=hexa.PREC_MIXED, nrhs=nrhs) as s:
+        s.set_problem(p.fixed, f.ravel())
+        s.set_density(p.rho)
+        ua, ra, sta = s.solve(**kw)
+    with hexa.Solver(nx, ny, nz, E0=1.0, Emin=1e-9, penal=3.0, precision=hexa.PREC_MIXED, nrhs=nrhs) as s:
+        xf, it, rel_ = s.filter(p.rho, 1.5, tol=1e-8, max_iter=200, check_every=1)
+        assert it > 0 and np.all(np.isfinite(xf))
+        s.set_problem(p.fixed, f.ravel())
+        s.set_density(p.rho)
+        ub, rb, stb = s.solve(**kw)
+    assert sta == stb == hexa.OK
+    assert np.array_equal(ua, ub)
+    assert [r["iterations"] for r in ra] == [r["iterations"] for r in rb]
+
+
 @pytest.mark.parametrize("grid", [(8, 4, 4), (7, 5, 3)])
 def test_reference_two_level_preconditioner_matches_oracle(grid):
     """HEXA_PRECOND_REF_MG (GPUGeometricMultigrid.jl:253-346) applied on the

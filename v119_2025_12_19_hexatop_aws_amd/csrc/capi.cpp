@@ -529,7 +529,7 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
             launch_pcg_alpha(st, h->d_state, h->d_partials, cpu ? 0 : 1);
         }
         launch_pcg_update(st, nd, h->d_state, defer_x ? nullptr : h->d_x, h->d_p, h->d_r, h->d_Ap,
-                          pk == 0 ? h->d_minv : nullptr, h->d_partials, r32);
+                          pk == 0 ? h->d_minv : nullptr, h->d_partials, h->d_upart, r32);
         if (pk == 2) {
             // z = M r and (r, z) for the standard beta (GPUSolver.jl:409-417)
             apply_refmg(h, h->d_r, h->d_z);
@@ -872,7 +872,7 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
         launch_pcg_alpha_n(st, h->d_state, h->d_dotpart, nq, 0, h->d_bscr, nb, pstride);
         // (mixed: x += alpha p is deferred to the p update, which reads p anyway)
         launch_pcg_update(st, nd, h->d_state, mx ? nullptr : h->d_x, h->d_p, h->d_r, h->d_Ap, nullptr, h->d_bpart,
-                          r32, nb, vs, PS);
+                          h->d_upart, r32, nb, vs, PS);
         launch_pcg_check(st, h->d_state, h->d_bpart, 1, 0, 1, 0, nb, PS);
         // beta as in the single solve: from the V-cycle's last smoothing
         // sweep's (r, z) partials where it forms them
@@ -1039,6 +1039,19 @@ int32_t solve_batched(hexa_handle* h, double* u_inout, const hexa_solve_params* 
 
 // =================================================================== C ABI
 namespace hexa {
+// The reduction, scalar and PCG-state buffers that the solver, the filter
+// and the element kernels share, at the sizes of the largest user (the
+// batched solve of nrhs load cases), whichever call allocates them first.
+int32_t ensure_shared_buffers(hexa_handle* h) {
+    const int nb = std::max(1, h->opt.nrhs);
+    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
+    // the PCG update's per-block sums (two per block and load case)
+    HX_TRY(dalloc(h, &h->d_upart, (size_t)(2 * pcg_update_blocks(3 * h->g.nn)) * nb));
+    HX_TRY(dalloc(h, &h->d_scal, 16 + (size_t)nb));
+    HX_TRY(dalloc(h, &h->d_state, (size_t)nb));
+    return HEXA_OK;
+}
+
 int32_t ensure_solver_vectors(hexa_handle* h) {
     const size_t nd = (size_t)(3 * h->g.nn);
     // with several load cases the iterate vectors hold a batch of them,
@@ -1054,14 +1067,12 @@ int32_t ensure_solver_vectors(hexa_handle* h) {
     HX_TRY(dalloc(h, &h->d_best, nd));
     HX_TRY(dalloc(h, &h->d_b, nv));
     HX_TRY(dalloc(h, &h->d_diag, nd));
-    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
+    HX_TRY(ensure_shared_buffers(h));
     // (p, Ap) partials of the FP64 operator and (r, z) partials of the
     // V-cycle's FP32 or FP64 last sweep, which may tile the grid differently
     HX_TRY(dalloc(h, &h->d_dotpart,
                   (size_t)std::max<int64_t>({1, elastic_rw_partials(h->g, sizeof(double)),
                                              elastic_rw_partials(h->g, sizeof(float))}) * nb));
-    HX_TRY(dalloc(h, &h->d_scal, 16 + (size_t)nb));
-    HX_TRY(dalloc(h, &h->d_state, (size_t)nb));
     if (nb > 1) {
         HX_TRY(dalloc(h, &h->d_bpart, (size_t)BPART_STRIDE * nb));
         HX_TRY(dalloc(h, &h->d_bscr, (size_t)RED_BLOCKS * nb));
@@ -1574,7 +1585,8 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
     const float* z32 = mx ? (const float*)l0.x : nullptr;
     if (mx) {
         HX_TRY(timed("pcg update (r, r32)", 0, 28.0 * nd, [&] {
-            launch_pcg_update(st, nd, h->d_state, nullptr, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials, r32);
+            launch_pcg_update(st, nd, h->d_state, nullptr, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials,
+                              h->d_upart, r32);
         }));
         HX_TRY(timed("pcg flex dots z32 (no last sweep)", 0, 20.0 * nd, [&] {
             launch_pcg_flex_dots_z32(st, nd, h->d_state, h->d_z, z32, h->d_Ap, h->d_partials);
@@ -1584,7 +1596,8 @@ int32_t hexa_kernel_report(hexa_handle* h, int32_t reps, hexa_kernel_stat* out, 
         }));
     } else {
         HX_TRY(timed("pcg update (x, r)", 0, 48.0 * nd, [&] {
-            launch_pcg_update(st, nd, h->d_state, h->d_best, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials);
+            launch_pcg_update(st, nd, h->d_state, h->d_best, h->d_p, h->d_z, h->d_Ap, nullptr, h->d_partials,
+                              h->d_upart);
         }));
         HX_TRY(timed("pcg flexible dots (no last sweep)", 0, 24.0 * nd, [&] {
             launch_pcg_flex_dots(st, nd, h->d_state, h->d_z, h->d_best, h->d_Ap, h->d_partials);

@@ -31,9 +31,7 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
     HX_TRY(dalloc(h, &h->d_hp, (size_t)nn));
     HX_TRY(dalloc(h, &h->d_hAp, (size_t)nn));
     HX_TRY(dalloc(h, &h->d_hdinv, (size_t)nn));
-    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
-    HX_TRY(dalloc(h, &h->d_scal, 16));
-    HX_TRY(dalloc(h, &h->d_state, 1));
+    HX_TRY(ensure_shared_buffers(h));
     double K8[64], M8[64], Ae[64 + 27 + 1];
     helmholtz_element(h->grid.dx, h->grid.dy, h->grid.dz, h->opt.helm_element == HEXA_HELM_BUG_COMPAT, K8, M8);
     for (int i = 0; i < 64; ++i) Ae[i] = radius * radius * K8[i] + M8[i];
@@ -71,7 +69,8 @@ int32_t filter_dev(hexa_handle* h, const double* d_in, double* d_out, double rad
         launch_helm_apply(st, g, h->d_Ae, h->d_hp, h->d_hAp);
         launch_pcg_pAp(st, nn, h->d_state, h->d_hp, h->d_hAp, h->d_partials);
         launch_pcg_alpha(st, h->d_state, h->d_partials, 0);
-        launch_pcg_update(st, nn, h->d_state, h->d_hx, h->d_hp, h->d_hr, h->d_hAp, h->d_hdinv, h->d_partials);
+        launch_pcg_update(st, nn, h->d_state, h->d_hx, h->d_hp, h->d_hr, h->d_hAp, h->d_hdinv, h->d_partials,
+                          h->d_upart);
         launch_pcg_check(st, h->d_state, h->d_partials, check_every, 0, 1, 1);
         launch_pcg_p_update(st, nn, h->d_state, h->d_hp, nullptr, h->d_hr, h->d_hdinv);
         return check_launch();
@@ -135,9 +134,7 @@ int32_t ensure_elem(hexa_handle* h) {
     HX_TRY(dalloc(h, &h->d_e4, ne));
     HX_TRY(dalloc(h, &h->d_e5, ne));
     HX_TRY(dalloc(h, &h->d_e6, ne));
-    HX_TRY(dalloc(h, &h->d_partials, (size_t)(MAX_RED_Q * RED_BLOCKS)));
-    HX_TRY(dalloc(h, &h->d_scal, 16));
-    HX_TRY(dalloc(h, &h->d_state, 1));
+    HX_TRY(ensure_shared_buffers(h));
     HX_TRY(dalloc_bytes(h, &h->d_bracket, (size_t)oc_bracket_bytes()));
     return HEXA_OK;
 }

@@ -98,6 +98,7 @@ struct hexa_handle {
     double *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_z = nullptr;
     double *d_minv = nullptr, *d_best = nullptr, *d_b = nullptr, *d_diag = nullptr, *d_guess = nullptr;
     double* d_partials = nullptr;
+    double* d_upart = nullptr;     // the PCG update's per-block sums (2 pcg_update_blocks(3 nn) per load case)
     double* d_dotpart = nullptr;   // (p, Ap) partials of the fused operator, then (r, z) partials of the
                                    // V-cycle's last smoothing sweep (per load case)
     double* d_bpart = nullptr;     // batched PCG: BPART_STRIDE partials per load case

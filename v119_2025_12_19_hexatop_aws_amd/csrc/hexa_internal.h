@@ -301,9 +301,13 @@ void launch_pcg_set_rz(hipStream_t s, PcgState* st, const double* partials);
 void launch_pcg_pAp(hipStream_t s, int64_t n, PcgState* st, const double* p, const double* Ap,
                     double* partials);
 void launch_pcg_alpha(hipStream_t s, PcgState* st, const double* partials, int breakdown_check);
+// Per-block partial sums of the update: upart holds 2 pcg_update_blocks(n)
+// values per right-hand side; they are summed to RED_BLOCKS range sums in
+// rows 0 (r, r) and 1 (r, minv r) of partials.
+int64_t pcg_update_blocks(int64_t n);
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32 = nullptr, int nb = 1,
-                       int64_t vs = 0, int ps = 0);
+                       const double* Ap, const double* minv, double* partials, double* upart, float* r32 = nullptr,
+                       int nb = 1, int64_t vs = 0, int ps = 0);
 // The residual test runs when k % check_every == 0 (check_every > 0), or
 // k == 1 with check_first; k is the state's iteration count.
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int check_every, int check_first,
@@ -329,6 +333,7 @@ void launch_pcg_best_copy(hipStream_t s, int64_t n, const PcgState* st, const do
 
 // Library-internal entry points shared by the C API files and the job server
 // (capi.cpp, capi_topopt.cpp).
+int32_t ensure_shared_buffers(hexa_handle* h);
 int32_t set_problem_impl(hexa_handle* h, const uint8_t* fixed_dof, const double* f, int64_t nsp,
                          const int64_t* sidx, const double* sval, const uint8_t* active_elem);
 int32_t oc_check(hexa_handle* h, const hexa_oc_params* p);

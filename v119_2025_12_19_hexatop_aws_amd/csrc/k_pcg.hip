@@ -182,22 +182,26 @@ __global__ __launch_bounds__(RED_THREADS) void k_beta(PcgState* st, const double
     if (threadIdx.x == 0) set_beta(st, rz);
 }
 
-// x += alpha p; r -= alpha Ap; partials: (r,r) and, when minv is given,
-// (r, minv r) for the Jacobi preconditioner.  r32, when given, receives the
-// new residual in FP32 (the mixed V-cycle's input).  With x null the x update
-// is deferred to the p update of the same iteration (which reads p anyway):
-// the update records the iteration st->k in st->x_iter.  Two entries per lane and step (16-byte
-// accesses); an odd last entry is handled by the first lane.
+// x += alpha p; r -= alpha Ap; per-block partial sums of (r,r) and, when
+// minv is given, (r, minv r) for the Jacobi preconditioner, in bpart[0..nblk)
+// and bpart[nblk..2 nblk) (summed to RED_BLOCKS range sums by
+// launch_pcg_update).  r32, when given, receives the new residual in FP32
+// (the mixed V-cycle's input).  With x null the x update is deferred to the
+// p update of the same iteration (which reads p anyway): the update records
+// the iteration st->k in st->x_iter.  One pair of entries per lane (16-byte
+// accesses); the lane after the last pair takes an odd last entry.  (Over
+// RED_BLOCKS blocks striding through the vector the same update took 18%
+// longer at 101.6M entries.)
 __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st, double* __restrict__ x,
                                                         const double* __restrict__ p, double* __restrict__ r,
                                                         const double* __restrict__ Ap,
                                                         const double* __restrict__ minv,
-                                                        double* __restrict__ partials,
-                                                        float* __restrict__ r32, int64_t vs, int ps) {
+                                                        double* __restrict__ bpart, int64_t nblk,
+                                                        float* __restrict__ r32, int64_t vs) {
     {
         const int rb = blockIdx.y;
         st += rb;
-        partials += (int64_t)rb * ps;
+        bpart += (int64_t)rb * 2 * nblk;
         const int64_t o = rb * vs;
         if (x) x += o;
         p += o;
@@ -210,8 +214,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st,
     const double a = st->alpha;
     double rr = 0.0, rz = 0.0;
     const int64_t n2 = n / 2;
-    const int64_t t0 = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x, ts = (int64_t)RED_BLOCKS * RED_THREADS;
-    for (int64_t i = t0; i < n2; i += ts) {
+    const int64_t i = blockIdx.x * (int64_t)RED_THREADS + threadIdx.x;
+    if (i < n2) {
         if (x) {
             double2 xv = reinterpret_cast<double2*>(x)[i];
             const double2 pv = reinterpret_cast<const double2*>(p)[i];
@@ -232,22 +236,21 @@ __global__ __launch_bounds__(RED_THREADS) void k_update(int64_t n, PcgState* st,
             rz += rv.x * (rv.x * mv.x);
             rz += rv.y * (rv.y * mv.y);
         }
-    }
-    if ((n & 1) && t0 == 0) {
-        const int64_t i = n - 1;
-        if (x) x[i] += a * p[i];
-        const double ri = r[i] - a * Ap[i];
-        r[i] = ri;
-        if (r32) r32[i] = (float)ri;
+    } else if (i == n2 && (n & 1)) {
+        const int64_t e = n - 1;
+        if (x) x[e] += a * p[e];
+        const double ri = r[e] - a * Ap[e];
+        r[e] = ri;
+        if (r32) r32[e] = (float)ri;
         rr += ri * ri;
-        if (minv) rz += ri * (ri * minv[i]);
+        if (minv) rz += ri * (ri * minv[e]);
     }
-    if (!x && t0 == 0) st->x_iter = st->k;
+    if (!x && i == 0) st->x_iter = st->k;
     rr = block_sum(rr, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = rr;
+    if (threadIdx.x == 0) bpart[blockIdx.x] = rr;
     if (minv) {
         rz = block_sum(rz, lds);
-        if (threadIdx.x == 0) partials[RED_BLOCKS + blockIdx.x] = rz;
+        if (threadIdx.x == 0) bpart[nblk + blockIdx.x] = rz;
     }
 }
 
@@ -471,10 +474,16 @@ void launch_pcg_beta_n(hipStream_t s, PcgState* st, const double* partials, int6
         k_beta<<<dim3(1, nb), RED_THREADS, 0, s>>>(st, scratch, RED_BLOCKS);
     }
 }
+int64_t pcg_update_blocks(int64_t n) { return ((n + 1) / 2 + RED_THREADS - 1) / RED_THREADS; }
 void launch_pcg_update(hipStream_t s, int64_t n, PcgState* st, double* x, const double* p, double* r,
-                       const double* Ap, const double* minv, double* partials, float* r32, int nb, int64_t vs,
-                       int ps) {
-    k_update<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, partials, r32, vs, ps);
+                       const double* Ap, const double* minv, double* partials, double* upart, float* r32, int nb,
+                       int64_t vs, int ps) {
+    const int64_t nblk = pcg_update_blocks(n);
+    k_update<<<dim3((unsigned)nblk, nb), RED_THREADS, 0, s>>>(n, st, x, p, r, Ap, minv, upart, nblk, r32, vs);
+    // the per-block sums to RED_BLOCKS range sums per quantity, in a fixed order
+    for (int q = 0; q < (minv ? 2 : 1); ++q)
+        k_sum_ranges<<<dim3(RED_BLOCKS, nb), RED_THREADS, 0, s>>>(st, upart + q * nblk, nblk,
+                                                                partials + q * RED_BLOCKS, 2 * nblk, ps);
 }
 void launch_pcg_check(hipStream_t s, PcgState* st, const double* partials, int check_every, int check_first,
                       int cpu_semantics, int jacobi, int nb, int ps) {
