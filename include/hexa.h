@@ -101,7 +101,7 @@ typedef struct {
     int32_t precision;         /* HEXA_PREC_*  (MG V-cycle arithmetic)      */
     int32_t matvec_kernel;     /* HEXA_MATVEC_*                            */
     int32_t mg_levels;         /* 0 = automatic                            */
-    int32_t mg_pre, mg_post;   /* damped-Jacobi sweeps                      */
+    int32_t mg_pre, mg_post;   /* damped-Jacobi sweeps, equal, 1..16        */
     double mg_omega;           /* Jacobi damping                            */
     int32_t nrhs;              /* load cases solved together (>= 1)        */
     int32_t diag_mode;         /* HEXA_DIAG_*                               */

@@ -50,6 +50,12 @@ def test_invalid_arguments_are_rejected_without_a_device():
     st = hexa.lib().hexa_create(C.byref(bad), None, C.byref(h))
     assert st == hexa.E_INVALID and "invalid grid" in hexa.last_error()
     assert hexa.lib().hexa_solve(None, None, None, None) == hexa.E_INVALID
+    # an unsymmetric V-cycle (other post- than pre-smoothing count) is refused,
+    # before any device call
+    g = hexa.Grid(4, 4, 4, 1.0, 1.0, 1.0)
+    for pre, post in ((2, 0), (2, 1), (1, 2)):
+        st = hexa.lib().hexa_create(C.byref(g), C.byref(hexa.default_options(mg_pre=pre, mg_post=post)), C.byref(h))
+        assert st == hexa.E_INVALID and "mg_post" in hexa.last_error()
 
 
 def test_estimate_counts_the_coarsest_inverse():

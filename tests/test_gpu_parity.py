@@ -71,6 +71,24 @@ def test_solve_matches_golden_direct(tag, precond, precision, kernel):
     assert abs(reps[0]["compliance"] - GOLD[f"solve_{tag}_compliance"][0]) <= 1e-10 * abs(f @ ug)
 
 
+@pytest.mark.parametrize("precision", [hexa.PREC_FP64, hexa.PREC_MIXED])
+@pytest.mark.parametrize("pre,post", [(1, 1), (3, 3), (4, 4)])
+def test_mg_sweep_counts_solve_to_golden(pre, post, precision):
+    """beta comes from the (r, z) sums of the V-cycle's last smoothing sweep;
+    sweep counts other than the default V(2,2) (V(1,1): the first sweep is
+    the only pre-sweep, and the fused two-sweep kernel is not used) solve to
+    the golden direct solution."""
+    say("mg sweeps", pre, post, precision)
+    grid, fixed, f, rho, ug = _golden_problem("b")
+    with hexa.Solver(*grid, E0=1.0, Emin=1e-9, penal=3.0, precision=precision, mg_pre=pre, mg_post=post) as s:
+        s.set_problem(fixed, f)
+        s.set_density(rho)
+        u, reps, st = s.solve(tol=1e-12, precond=hexa.PRECOND_MG, semantics=hexa.SEMANTICS_CPU, shift_factor=1e-10,
+                              max_iter=20000)
+    assert st == hexa.OK, reps[0]
+    assert rel(u[0], ug) < 1e-9
+
+
 def test_gpu_semantics_match_oracle_control_flow():
     say("gpu semantics")
     grid, fixed, f, rho, ug = _golden_problem("b")

@@ -510,6 +510,7 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
     // there, with CPU semantics (no best-iterate copy of x after the check),
     // x += alpha p moves from the update to the p update, which reads p anyway
     const bool defer_x = mx && cpu;
+    double* const vdots = h->d_dotpart;
     // One iteration.  Everything that changes between iterations (the
     // iteration count, the check schedule, the deferred x update) is decided
     // on the device, so the launches are the same every iteration.
@@ -539,13 +540,15 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
                          use_mg ? 0 : 1);
         // (copies only when the check improved the best residual)
         if (!cpu) launch_pcg_best_copy(st, nd, h->d_state, h->d_x, h->d_best);
-        // beta: the V-cycle's last smoothing sweep forms the (r, z) partials
-        // (with the FP32 residual it smooths, in the mixed V-cycle) and beta
-        // is the standard (r, z)_new / (r, z); without such a sweep, the
-        // flexible beta from a separate pass over r, z and Ap
+        // beta: the V-cycle (symmetric: as many sweeps after the coarse
+        // correction as before) ends with a smoothing sweep that forms the
+        // (r, z) partials (with the FP32 residual it smooths, in the mixed
+        // V-cycle), and beta is the standard (r, z)_new / (r, z); with the
+        // dense-Ke V-cycle kernels, the flexible beta from a separate pass
+        // over r, z and Ap
         double* const bscr = h->d_partials + (MAX_RED_Q - 1) * RED_BLOCKS;
         if (mx) {
-            const int nz = vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32, 1, h->d_dotpart);
+            const int nz = vcycle_level<float, uint16_t>(h, 0, r32, (float*)z32, 1, vdots);
             if (nz > 0) {
                 launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, bscr);
             } else {
@@ -554,7 +557,7 @@ int32_t run_attempt(hexa_handle* h, const hexa_solve_params* prm, int pk, double
             }
             launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, defer_x ? h->d_x : nullptr);
         } else if (use_mg) {
-            const int nz = apply_mg(h, h->d_r, h->d_z, 1, h->d_dotpart);
+            const int nz = apply_mg(h, h->d_r, h->d_z, 1, vdots);
             if (nz > 0) {
                 launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, bscr);
             } else {
@@ -860,6 +863,7 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
     const int64_t pstride = elastic_rw_partials(h->g);
     // the V-cycle's (r, z) partials (from its FP32 or FP64 level-0 kernels)
     const int64_t vstride = elastic_rw_partials(h->g, mx ? sizeof(float) : sizeof(double));
+    double* const vdots = h->d_dotpart;
     auto iteration = [&]() -> int32_t {
         // a load case's operator and V-cycle work stops once its solve ended
         HaltScope halt_scope(h, nb);
@@ -877,7 +881,7 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
         // beta as in the single solve: from the V-cycle's last smoothing
         // sweep's (r, z) partials where it forms them
         if (mx) {
-            const int nz = vcycle_level<float, uint16_t>(h, 0, r32, z32, nb, h->d_dotpart);
+            const int nz = vcycle_level<float, uint16_t>(h, 0, r32, z32, nb, vdots);
             if (nz > 0) {
                 launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, h->d_bscr, nb, vstride);
             } else {
@@ -886,7 +890,7 @@ int32_t run_attempt_batched(hexa_handle* h, const hexa_solve_params* prm, double
             }
             launch_pcg_p_update_z32(st, nd, h->d_state, h->d_p, z32, h->d_x, nb, vs);
         } else {
-            const int nz = apply_mg(h, h->d_r, h->d_z, nb, h->d_dotpart);
+            const int nz = apply_mg(h, h->d_r, h->d_z, nb, vdots);
             if (nz > 0) {
                 launch_pcg_beta_n(st, h->d_state, h->d_dotpart, nz, h->d_bscr, nb, vstride);
             } else {
@@ -1172,8 +1176,12 @@ int32_t hexa_create(const hexa_grid* grid, const hexa_options* opt, hexa_handle*
     if (opt) o = *opt; else hexa_default_options(&o);
     if (o.nrhs < 1 || o.nrhs > 64) { set_error("nrhs must be in [1, 64]"); return HEXA_E_INVALID; }
     if (!(o.mg_omega > 0.0 && o.mg_omega < 2.0)) { set_error("mg_omega must be in (0, 2)"); return HEXA_E_INVALID; }
-    if (o.mg_pre < 1 || o.mg_post < 0 || o.mg_pre > 16 || o.mg_post > 16) {
-        set_error("mg_pre must be in [1,16], mg_post in [0,16]");
+    // The V-cycle preconditions CG, which needs it symmetric positive: as
+    // many Jacobi sweeps after the coarse correction as before.  (On an
+    // 8x4x4 cantilever with the golden test's density, V(2,0) and V(2,1)
+    // ended in NaN with the flexible beta too.)
+    if (o.mg_pre < 1 || o.mg_pre > 16 || o.mg_post != o.mg_pre) {
+        set_error("mg_pre must be in [1,16] and mg_post equal to it (a symmetric V-cycle)");
         return HEXA_E_INVALID;
     }
     if (!(o.penal > 0) || o.E0 < o.Emin) { set_error("invalid material law"); return HEXA_E_INVALID; }
