@@ -182,7 +182,7 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE counters of a rocprofv3 --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the library has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
 
