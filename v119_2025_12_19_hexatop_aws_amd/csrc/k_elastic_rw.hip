@@ -680,8 +680,13 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
     if ((te && te[0] == '0') || (int64_t)g.nx * g.ny * g.nz < 32768 || !elastic_rw_supported(g, sizeof(T))) return;
     const int kind = sizeof(T) == 8 ? (int)TK_RW64 : (int)TK_RW32;
     hipEvent_t e0, e1;
+    int ncu = 0;   // compute units of the current device
     {
         DeviceOpGuard lock;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
         if (hipEventCreate(&e0) != hipSuccess) return;
         if (hipEventCreate(&e1) != hipSuccess) {
             (void)hipEventDestroy(e0);
@@ -715,10 +720,6 @@ void tune_elastic_rw(hipStream_t s, const Grid3& g, const FactCoef<T>& coef, con
         // of 44-plane chunks (243 blocks) took 0.055 ms against 0.064 ms
         // with the best power of two.
         {
-            int dev = 0, ncu = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
-                ncu = 256;
             const int64_t cap = (int64_t)ncu * (sizeof(T) == 8 ? 1 : 2);
             const int64_t txy = (int64_t)r4.tiles_x * r4.tiles_y;
             for (int rounds = 1; rounds <= 4; ++rounds) {
